@@ -1,0 +1,113 @@
+"""Host-side discretization: reference element, box mesh, geometry and face
+connectivity, packed into the arrays the device plan consumes.
+
+Stands in for grudge's ``DiscretizationCollection`` built in
+get_{wave,euler,maxwell}_rhs (equations/wave.py:261-270,
+equations/euler.py:157-166, equations/maxwell.py:250).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from functools import cached_property
+
+import numpy as np
+
+from actx_dgfem_suite_amd.discretization.mesh import (
+    BoxMesh,
+    Connectivity,
+    Geometry,
+    compute_connectivity,
+    compute_exterior_map,
+    compute_geometry,
+    element_nodes,
+    generate_box_mesh,
+)
+from actx_dgfem_suite_amd.discretization.refelem import (
+    NFACES,
+    ReferenceElement,
+    np_for_order,
+    nfp_for_order,
+    reference_element,
+)
+
+GEOMETRY_STRIDE = 25
+
+
+@dataclass
+class Discretization:
+    mesh: BoxMesh
+    ref: ReferenceElement
+
+    @property
+    def order(self) -> int:
+        return self.ref.order
+
+    @property
+    def dim(self) -> int:
+        return 3
+
+    @property
+    def nelements(self) -> int:
+        return self.mesh.nelements
+
+    @property
+    def np(self) -> int:
+        return self.ref.np
+
+    @property
+    def nfp(self) -> int:
+        return self.ref.nfp
+
+    @property
+    def ndofs(self) -> int:
+        return self.nelements * self.np
+
+    @cached_property
+    def geometry(self) -> Geometry:
+        return compute_geometry(self.mesh)
+
+    @cached_property
+    def connectivity(self) -> Connectivity:
+        return compute_connectivity(self.mesh)
+
+    @cached_property
+    def ext_map(self) -> np.ndarray:
+        return compute_exterior_map(self.mesh, self.connectivity, self.ref)
+
+    @cached_property
+    def nodes(self) -> np.ndarray:
+        """(3, K, Np) physical node coordinates."""
+        return element_nodes(self.mesh, self.ref)
+
+    def packed_geometry(self) -> np.ndarray:
+        """(K, 25): rx[x][r] (9), then per face (n_x, n_y, n_z, Fscale)."""
+        g = self.geometry
+        k = self.nelements
+        out = np.empty((k, GEOMETRY_STRIDE))
+        out[:, :9] = g.rx.reshape(k, 9)
+        face = np.concatenate([g.normals, g.fscale[:, :, None]], axis=2)  # (K,4,4)
+        out[:, 9:] = face.reshape(k, 16)
+        return out
+
+    def packed_ext_map(self) -> np.ndarray:
+        m = self.ext_map
+        if m.max() >= 2**31 - 1 or m.min() < -(2**31):
+            raise NotImplementedError("mesh too large for 32-bit node indices")
+        return np.ascontiguousarray(m, dtype=np.int32)
+
+
+def make_box_discretization(order: int, nel_1d: int, a=(-1.0, -1.0, -1.0),
+                            b=(1.0, 1.0, 1.0)) -> Discretization:
+    return Discretization(mesh=generate_box_mesh(nel_1d, a, b), ref=reference_element(order))
+
+
+__all__ = [
+    "NFACES",
+    "Discretization",
+    "GEOMETRY_STRIDE",
+    "make_box_discretization",
+    "np_for_order",
+    "nfp_for_order",
+    "reference_element",
+]

@@ -1,0 +1,221 @@
+"""Box meshes of affine tetrahedra, face connectivity and per-element geometry.
+
+Host-side setup that stands in for ``meshmode.mesh.generation.
+generate_regular_rect_mesh`` and the geometry grudge derives from it
+(equations/wave.py:248-270, equations/euler.py:153-166,
+equations/maxwell.py:246-250): a box split into ``n^3`` hexahedra, each cut
+into 6 tetrahedra, with affine (per-element constant) inverse Jacobians,
+outward unit normals and face area elements.
+
+Element ordering keeps neighbours close in memory: hexahedron (i, j, k) with
+k fastest, then the 6 tetrahedra of that hexahedron.  Each hexahedron uses the
+Kuhn (Freudenthal) split along its main diagonal, which is conforming across
+hexahedra on a regular grid.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from itertools import permutations
+
+import numpy as np
+
+from actx_dgfem_suite_amd.discretization.refelem import (
+    FACE_VERTICES,
+    NFACES,
+    REF_NORMALS,
+    ReferenceElement,
+)
+
+# Kuhn split: tetrahedron for axis permutation p has vertices
+# 0, e_p0, e_p0+e_p1, 1 (corner offsets in the unit cube).
+_KUHN_PERMS = tuple(permutations(range(3)))
+
+
+@dataclass(frozen=True)
+class BoxMesh:
+    vertices: np.ndarray    # (nverts, 3)
+    elements: np.ndarray    # (K, 4) int64, positively oriented
+    nel_1d: int
+    a: tuple[float, float, float]
+    b: tuple[float, float, float]
+
+    @property
+    def nelements(self) -> int:
+        return self.elements.shape[0]
+
+
+def generate_box_mesh(nel_1d: int, a=(-1.0, -1.0, -1.0), b=(1.0, 1.0, 1.0)
+                      ) -> BoxMesh:
+    n = int(nel_1d)
+    if n < 1:
+        raise ValueError("nel_1d must be >= 1")
+    a = tuple(float(x) for x in np.broadcast_to(np.asarray(a, float), (3,)))
+    b = tuple(float(x) for x in np.broadcast_to(np.asarray(b, float), (3,)))
+    axes = [np.linspace(a[d], b[d], n + 1) for d in range(3)]
+    gx, gy, gz = np.meshgrid(*axes, indexing="ij")
+    vertices = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+
+    def vid(i, j, k):
+        return (i * (n + 1) + j) * (n + 1) + k
+
+    hi, hj, hk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    hi, hj, hk = hi.ravel(), hj.ravel(), hk.ravel()
+    tets = np.empty((n ** 3, 6, 4), dtype=np.int64)
+    for t, p in enumerate(_KUHN_PERMS):
+        corner = np.zeros(3, dtype=np.int64)
+        cols = []
+        for step in range(4):
+            cols.append(vid(hi + corner[0], hj + corner[1], hk + corner[2]))
+            if step < 3:
+                corner = corner.copy()
+                corner[p[step]] += 1
+        tets[:, t, :] = np.stack(cols, axis=1)
+    elements = tets.reshape(-1, 4)
+    # orient positively: det[x1-x0, x2-x0, x3-x0] > 0
+    x = vertices[elements]
+    det = np.linalg.det(np.stack([x[:, 1] - x[:, 0], x[:, 2] - x[:, 0],
+                                  x[:, 3] - x[:, 0]], axis=2))
+    neg = det < 0
+    elements[neg] = elements[neg][:, [0, 2, 1, 3]]
+    return BoxMesh(vertices=vertices, elements=elements, nel_1d=n, a=a, b=b)
+
+
+@dataclass(frozen=True)
+class Geometry:
+    """Affine per-element geometry.
+
+    ``rx[e, x, r] = d r_r / d x_x`` (inverse Jacobian), ``jac[e]`` = volume
+    Jacobian, ``normals[e, f]`` outward unit normals, ``sjac[e, f]`` face area
+    element relative to the parameter triangle of area 2, ``fscale = sjac/jac``.
+    """
+    rx: np.ndarray        # (K, 3, 3)
+    jac: np.ndarray       # (K,)
+    normals: np.ndarray   # (K, 4, 3)
+    sjac: np.ndarray      # (K, 4)
+
+    @property
+    def fscale(self) -> np.ndarray:
+        return self.sjac / self.jac[:, None]
+
+
+def compute_geometry(mesh: BoxMesh) -> Geometry:
+    x = mesh.vertices[mesh.elements]                    # (K, 4, 3)
+    # columns d x / d r, d x / d s, d x / d t
+    jm = 0.5 * np.stack([x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]],
+                        axis=2)                         # (K, 3(x), 3(r))
+    jac = np.linalg.det(jm)
+    if np.any(jac <= 0):
+        raise RuntimeError("non-positive element Jacobian")
+    inv = np.linalg.inv(jm)                             # (K, 3(r), 3(x))
+    rx = np.transpose(inv, (0, 2, 1)).copy()            # rx[e, x, r]
+    # outward normal direction of face f: J^{-T} n_ref
+    nrm = np.einsum("erx,fr->efx", inv, REF_NORMALS)
+    nrm /= np.linalg.norm(nrm, axis=2)[:, :, None]
+    sjac = np.empty((mesh.nelements, NFACES))
+    for f, (i0, i1, i2) in enumerate(FACE_VERTICES):
+        cr = np.cross(x[:, i1] - x[:, i0], x[:, i2] - x[:, i0])
+        sjac[:, f] = 0.5 * np.linalg.norm(cr, axis=1) / 2.0
+    return Geometry(rx=rx, jac=jac, normals=nrm, sjac=sjac)
+
+
+@dataclass(frozen=True)
+class Connectivity:
+    """Face neighbours.  ``nbr_elem[e, f] < 0`` marks a boundary face."""
+    nbr_elem: np.ndarray   # (K, 4) int64
+    nbr_face: np.ndarray   # (K, 4) int64
+    perm: np.ndarray       # (K, 4) int64: index into itertools.permutations(range(3))
+
+
+def compute_connectivity(mesh: BoxMesh) -> Connectivity:
+    k = mesh.nelements
+    fv = np.asarray(FACE_VERTICES)
+    gverts = mesh.elements[:, fv]                       # (K, 4, 3)
+    keys = np.sort(gverts, axis=2).reshape(-1, 3)
+    order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+    sk = keys[order]
+    same = np.all(sk[1:] == sk[:-1], axis=1)
+    nbr = np.full(k * NFACES, -1, dtype=np.int64)
+    i = np.nonzero(same)[0]
+    nbr[order[i]] = order[i + 1]
+    nbr[order[i + 1]] = order[i]
+    nbr = nbr.reshape(k, NFACES)
+    nbr_elem = np.where(nbr >= 0, nbr // NFACES, -1)
+    nbr_face = np.where(nbr >= 0, nbr % NFACES, -1)
+    # vertex permutation p with h[p[m]] == g[m] (g: own face verts, h: neighbour's)
+    perm = np.zeros((k, NFACES), dtype=np.int64)
+    inter = nbr_elem >= 0
+    g = gverts[inter]
+    h = gverts[nbr_elem[inter], nbr_face[inter]]
+    pidx = np.full(g.shape[0], -1, dtype=np.int64)
+    for ip, p in enumerate(permutations(range(3))):
+        ok = np.all(h[:, list(p)] == g, axis=1)
+        pidx[ok] = ip
+    if np.any(pidx < 0):
+        raise RuntimeError("face vertex mismatch")
+    perm[inter] = pidx
+    return Connectivity(nbr_elem=nbr_elem, nbr_face=nbr_face, perm=perm)
+
+
+def _face_barycentrics(ref: ReferenceElement) -> np.ndarray:
+    """bary[f, j, m]: barycentric of face node j w.r.t. face f's vertex m."""
+    from actx_dgfem_suite_amd.discretization.refelem import REF_VERTICES
+    out = np.empty((NFACES, ref.nfp, 3))
+    for f, vs in enumerate(FACE_VERTICES):
+        pts = ref.nodes[ref.face_nodes[f]]
+        a = np.stack([REF_VERTICES[vs[1]] - REF_VERTICES[vs[0]],
+                      REF_VERTICES[vs[2]] - REF_VERTICES[vs[0]]], axis=1)
+        lam12, *_ = np.linalg.lstsq(a, (pts - REF_VERTICES[vs[0]]).T, rcond=None)
+        out[f, :, 1:] = lam12.T
+        out[f, :, 0] = 1.0 - lam12.sum(axis=0)
+    return out
+
+
+def face_node_permutation_table(ref: ReferenceElement) -> np.ndarray:
+    """tbl[f, f2, p, j] = neighbour face-node index matching own node j."""
+    bary = _face_barycentrics(ref)
+    perms = list(permutations(range(3)))
+    tbl = np.empty((NFACES, NFACES, len(perms), ref.nfp), dtype=np.int64)
+    for f in range(NFACES):
+        for f2 in range(NFACES):
+            for ip, p in enumerate(perms):
+                # own vertex m is neighbour vertex p[m]
+                target = np.empty_like(bary[f])
+                target[:, list(p)] = bary[f]
+                d = np.abs(target[:, None, :] - bary[f2][None, :, :]).sum(axis=2)
+                j2 = d.argmin(axis=1)
+                if np.any(d[np.arange(ref.nfp), j2] > 1e-8):
+                    raise RuntimeError("face nodes are not symmetric under "
+                                       "vertex permutation")
+                tbl[f, f2, ip] = j2
+    return tbl
+
+
+def compute_exterior_map(mesh: BoxMesh, conn: Connectivity, ref: ReferenceElement
+                         ) -> np.ndarray:
+    """Flat exterior-node index for every face node, shape (K, 4*Nfp), int64.
+
+    Interior faces: ``e2 * Np + face_nodes[f2, j2]``.  Boundary faces:
+    ``-(e * Np + face_nodes[f, j]) - 1`` (the interior node, negated), so one
+    int32 array carries both the gather index and the boundary flag.
+    """
+    k, npn, nfp = mesh.nelements, ref.np, ref.nfp
+    tbl = face_node_permutation_table(ref)
+    e = np.arange(k)[:, None, None]
+    own = e * npn + ref.face_nodes[None, :, :]          # (K, 4, Nfp)
+    e2 = conn.nbr_elem[:, :, None]
+    f2 = np.maximum(conn.nbr_face, 0)[:, :, None]
+    j2 = tbl[np.arange(NFACES)[None, :, None], f2, conn.perm[:, :, None],
+             np.arange(nfp)[None, None, :]]
+    ext = e2 * npn + ref.face_nodes[f2, j2]
+    vmap = np.where(e2 >= 0, ext, -own - 1)
+    return vmap.reshape(k, NFACES * nfp)
+
+
+def element_nodes(mesh: BoxMesh, ref: ReferenceElement) -> np.ndarray:
+    """Physical node coordinates, shape (3, K, Np) (meshmode ``nodes()`` order)."""
+    x = mesh.vertices[mesh.elements]                    # (K, 4, 3)
+    r, s, t = ref.nodes.T
+    lam = np.stack([-(1.0 + r + s + t) / 2.0, (1.0 + r) / 2.0,
+                    (1.0 + s) / 2.0, (1.0 + t) / 2.0], axis=1)   # (Np, 4)
+    return np.einsum("nv,evx->xen", lam, x)

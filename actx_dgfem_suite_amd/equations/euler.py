@@ -1,0 +1,43 @@
+"""Compressible Euler operator: builder mirroring ``get_euler_rhs``
+(equations/euler.py:140-186 of the reference).
+
+The operator is upstream ``grudge.models.euler.EulerOperator`` with
+``flux_type="lf"``, ``InviscidWallBC`` on all walls and gamma = 1.4
+(equations/euler.py:168-174): ``q_t = M^{-1}[weak_div F(q) - face_mass(F*)]``
+with ``F* = {F}.n + lam/2 (q- - q+)``, ``lam = max(|u| + c)`` over both
+traces, and walls reflecting the normal momentum.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from actx_dgfem_suite_amd.arraycontext import DGOperator
+from actx_dgfem_suite_amd.containers import ConservedEulerField, obj_array
+from actx_dgfem_suite_amd.equations._common import box_discretization, make_rhs
+
+
+def acoustic_pulse_condition(nodes: np.ndarray, gamma: float = 1.4):
+    """equations/euler.py:53-137: rho = 1, u = 0, E = p0/(gamma-1) plus a
+    Gaussian pulse exp(-r^2 / (2 w^2)) with w = 0.1 at the origin."""
+    r2 = sum(nodes[i] ** 2 for i in range(3))
+    mass = np.ones_like(r2)
+    energy = 1.0 / (gamma - 1.0) + np.exp(-0.5 * r2 / 0.1 ** 2)
+    mom = [np.zeros_like(r2) for _ in range(3)]
+    return mass, energy, mom
+
+
+def get_euler_rhs(*, actx, dim: int, order: int, ndofs: int | None = None,
+                  nel_1d: int | None = None, a=(-0.5,) * 3, b=(0.5,) * 3,
+                  initial_condition=None, gamma: float = 1.4):
+    """Returns ``(rhs, (t, fields))`` with ``rhs(t, q) -> q_t``; ``t`` is
+    unused, as in the reference (``np.float64(0.5)``, euler.py:186)."""
+    discr = box_discretization(dim, order, ndofs, nel_1d, a, b)
+    if initial_condition is None:
+        mass, energy, mom = acoustic_pulse_condition(discr.nodes, gamma)
+    else:
+        mass, energy, *mom = initial_condition(discr.nodes)
+    fields = actx.from_numpy(ConservedEulerField(mass=mass, energy=energy,
+                                                 momentum=obj_array(mom)))
+    op = DGOperator("euler", discr, float(gamma), takes_time=True)
+    return make_rhs(actx, op), (np.float64(0.5), fields)

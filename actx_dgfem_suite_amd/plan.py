@@ -1,0 +1,141 @@
+"""Device plan for one DG operator: owns the library plan handle and launches
+the RHS and LSERK4 kernels on torch device memory.
+
+torch supplies device memory and the current stream; all arithmetic happens in
+``libdgfem.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+
+import numpy as np
+import torch
+
+from actx_dgfem_suite_amd import _lib
+from actx_dgfem_suite_amd.discretization import Discretization
+
+NFIELDS = {"wave": 4, "euler": 5, "maxwell": 6}
+
+
+def _dptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class DGPlan:
+    """Plan for ``equation`` on ``discr`` with scalar ``param``.
+
+    ``param`` is the wave speed c (wave), gamma (Euler) or the flux_type alpha
+    (Maxwell), matching equations/wave.py:279, equations/euler.py:151 and
+    equations/maxwell.py:252-254.
+    """
+
+    def __init__(self, discr: Discretization, equation: str, param: float,
+                 device: torch.device | int | None = None):
+        if equation not in _lib.EQUATION_IDS:
+            raise NotImplementedError(f"equation '{equation}' not supported")
+        if not torch.cuda.is_available():
+            raise RuntimeError("DGPlan needs a ROCm GPU (no CPU fallback)")
+        self.lib = _lib.load_library()
+        self.discr = discr
+        self.equation = equation
+        self.param = float(param)
+        self.nfields = NFIELDS[equation]
+        self.device = torch.device("cuda", torch.cuda.current_device()
+                                   if device is None else torch.device(device).index or 0)
+        ref = discr.ref
+        self._host = {
+            "d_weak": np.ascontiguousarray(ref.d_weak, dtype=np.float64),
+            "d_strong": np.ascontiguousarray(ref.d_strong, dtype=np.float64),
+            "lift": np.ascontiguousarray(ref.lift, dtype=np.float64),
+            "face_nodes": np.ascontiguousarray(ref.face_nodes, dtype=np.int32),
+            "geometry": np.ascontiguousarray(discr.packed_geometry(), dtype=np.float64),
+            "ext_map": discr.packed_ext_map(),
+        }
+        h = self._host
+        desc = _lib.DgPlanDesc()
+        desc.equation = _lib.EQUATION_IDS[equation]
+        desc.order = discr.order
+        desc.nelements = discr.nelements
+        desc.d_weak = _dptr(h["d_weak"], ctypes.c_double)
+        desc.d_strong = _dptr(h["d_strong"], ctypes.c_double)
+        desc.lift = _dptr(h["lift"], ctypes.c_double)
+        desc.face_nodes = _dptr(h["face_nodes"], ctypes.c_int32)
+        desc.geometry = _dptr(h["geometry"], ctypes.c_double)
+        desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
+        desc.params[0] = self.param
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib, self.lib.dg_plan_create(ctypes.byref(desc),
+                                                         ctypes.byref(handle)),
+                       "dg_plan_create")
+        self._handle = handle
+        del self._host  # the plan holds device copies
+
+    # {{{ validation
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.discr.nelements, self.discr.np)
+
+    def _check_fields(self, fields: Sequence[torch.Tensor], what: str) -> None:
+        if len(fields) != self.nfields:
+            raise ValueError(f"{what}: expected {self.nfields} fields, got {len(fields)}")
+        for k, t in enumerate(fields):
+            if not isinstance(t, torch.Tensor):
+                raise TypeError(f"{what}[{k}]: expected torch.Tensor, got {type(t)}")
+            if t.device != self.device:
+                raise ValueError(f"{what}[{k}] on {t.device}, plan on {self.device}")
+            if t.dtype != torch.float64:
+                raise TypeError(f"{what}[{k}]: expected float64, got {t.dtype}")
+            if tuple(t.shape) != self.shape or not t.is_contiguous():
+                raise ValueError(f"{what}[{k}]: expected contiguous {self.shape}, "
+                                 f"got {tuple(t.shape)}")
+
+    # }}}
+
+    def empty_fields(self) -> list[torch.Tensor]:
+        return [torch.empty(self.shape, dtype=torch.float64, device=self.device)
+                for _ in range(self.nfields)]
+
+    def rhs(self, q: Sequence[torch.Tensor], out: Sequence[torch.Tensor] | None = None,
+            precision: str = "fp64") -> list[torch.Tensor]:
+        """Launch one RHS evaluation on the current stream; returns ``out``."""
+        if precision not in _lib.PRECISION_IDS:
+            raise NotImplementedError(f"precision '{precision}'")
+        self._check_fields(q, "q")
+        if out is None:
+            out = self.empty_fields()
+        else:
+            self._check_fields(out, "out")
+        pin = _lib.pointer_array([t.data_ptr() for t in q])
+        pout = _lib.pointer_array([t.data_ptr() for t in out])
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_rhs(self._handle, _lib.PRECISION_IDS[precision],
+                                             pin, pout, ctypes.c_void_p(stream)), "dg_rhs")
+        return list(out)
+
+    def lserk_stage(self, a: float, b: float, dt: float, q: Sequence[torch.Tensor],
+                    res: Sequence[torch.Tensor], rhs: Sequence[torch.Tensor]) -> None:
+        for name, f in (("q", q), ("res", res), ("rhs", rhs)):
+            self._check_fields(f, name)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_lserk_stage(
+            self._handle, float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in q]),
+            _lib.pointer_array([t.data_ptr() for t in res]),
+            _lib.pointer_array([t.data_ptr() for t in rhs]),
+            ctypes.c_void_p(stream)), "dg_lserk_stage")
+
+    def close(self) -> None:
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self.lib.dg_plan_destroy(h)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,203 @@
+"""Benchmark: DG right-hand-side evaluations per second on one MI355X.
+
+Default workload: linear acoustic wave, box [-1,1]^3 as 30^3 hexahedra x 6
+tetrahedra (162,000 tets), P3, 4 fields of 3.24M nodes each, FP64, upwind
+flux; 1000 timed RHS evaluations.  A "step" is one RHS evaluation
+``rhs = L(q)`` with ``q`` already resident in HBM.
+
+Prints one JSON line with the metric, the roofline position of the fused RHS
+kernel (timed with HIP events around every launch on the launch stream) and a
+CPU baseline (the numpy oracle on a bounded sample of the same workload).
+
+    python bench.py [--steps 1000] [--warmup 20] [--workload wave_p3]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
+
+
+def _parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--workload", default="wave_p3")
+    p.add_argument("--precision", default="fp64")
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="approximate CPU time budget of the cpu_baseline sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="also run a plain device copy of known size (HBM counter "
+                        "calibration under rocprofv3 --pmc)")
+    return p.parse_args()
+
+
+def cpu_baseline(workload, seconds: float):
+    """Time the numpy oracle on a bounded sample: the same equation and order
+    on a smaller box, reported as evaluations/s of the full workload (scaled
+    by the element count)."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from actx_dgfem_suite_amd.discretization import make_box_discretization
+    from actx_dgfem_suite_amd.workloads import initial_state
+    from oracle import dg_oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    nel = max(2, min(workload.nel_1d, 10))
+    d = make_box_discretization(workload.order, nel)
+    od = dg_oracle.build_discretization(d.mesh.vertices, d.mesh.elements, d.ref.nodes)
+    q = initial_state(workload, d)
+
+    def one():
+        if workload.equation == "wave":
+            dg_oracle.wave_rhs(od, q[0], q[1:4], c=workload.param)
+        elif workload.equation == "euler":
+            dg_oracle.euler_rhs(od, q, gamma=workload.param)
+        else:
+            dg_oracle.maxwell_rhs(od, q, alpha=workload.param)
+
+    with threadpool_limits(threads):
+        one()
+        n, t0 = 0, time.perf_counter()
+        while True:
+            one()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or n >= 10000:
+                break
+    full_k = 6 * workload.nel_1d ** 3
+    per_eval_full = el / n * full_k / d.nelements
+    return {
+        "value": 1.0 / per_eval_full,
+        "unit": "RHS evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"numpy oracle, {workload.equation} P{workload.order} on "
+                   f"{nel}^3x6={d.nelements} tets, {n} evals in {el:.1f} s, "
+                   f"scaled by elements to {full_k} tets"),
+    }
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU path is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from actx_dgfem_suite_amd.plan import DGPlan
+    from actx_dgfem_suite_amd.workloads import (
+        WORKLOADS,
+        algorithmic_bytes,
+        algorithmic_flops,
+        initial_state,
+    )
+
+    w = WORKLOADS[args.workload]
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    d = w.discretization()
+    plan = DGPlan(d, w.equation, w.param, device=dev)
+    q = [torch.from_numpy(np.ascontiguousarray(f)).to(dev) for f in initial_state(w, d)]
+    out = plan.empty_fields()
+    nf = len(q)
+    k, npn, nfp = d.nelements, d.np, d.nfp
+    print(f"[bench] {w.name}: K={k} Np={npn} fields={nf} setup done", file=sys.stderr,
+          flush=True)
+
+    for _ in range(args.warmup):
+        plan.rhs(q, out, precision=args.precision)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream(dev)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        starts[s].record(stream)
+        plan.rhs(q, out, precision=args.precision)
+        ends[s].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = sum(a.elapsed_time(b) for a, b in zip(starts, ends)) / args.steps
+
+    if args.copy_probe:
+        n = 256 * 1024 * 1024 // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {n * 8} bytes read + written x3", file=sys.stderr)
+
+    b_alg = algorithmic_bytes(k, npn, nfp, nf)
+    f_alg = algorithmic_flops(w.equation, k, npn, nfp, nf)
+    t_kernel = kernel_ms * 1e-3
+    achieved = b_alg / t_kernel / 1e9
+    t_roof = max(b_alg / (HBM_PEAK_GBS * 1e9), f_alg / (FP64_PEAK_TFLOPS * 1e12))
+    ms_per_step = wall / args.steps * 1e3
+
+    result = {
+        "metric": "DG RHS evals/s",
+        "value": args.steps / wall,
+        "unit": "RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "equation": w.equation,
+            "order": w.order,
+            "mesh": f"[-1,1]^3, {w.nel_1d}^3 hex x 6 tets",
+            "nelements": k,
+            "dofs_per_field": k * npn,
+            "fields": nf,
+            "precision": args.precision,
+        },
+        "gflops": f_alg / t_kernel / 1e9,
+        "gdofs_per_s": k * npn / t_kernel / 1e9,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"rhs_fp64_kernel<{w.equation},P{w.order}>",
+            "kernel_us": kernel_ms * 1e3,
+            "algorithmic_bytes": b_alg,
+            "algorithmic_flops": f_alg,
+            "combined_frac": t_roof / t_kernel,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline sample ...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,118 @@
+/*
+ * dgfem.h -- C ABI of libdgfem.so, the gfx950 DG-FEM right-hand-side engine.
+ *
+ * The reference (kaushikcfd/actx_dgfem_suite) is pure Python: its hot path is
+ * `compiled_rhs = actx.compile(rhs); compiled_rhs(*args)` (run/__main__.py:74-76,
+ * :109, :126), where `rhs` is the wave / Euler / Maxwell operator built by
+ * get_{wave,euler,maxwell}_rhs (equations/wave.py:239-285,
+ * equations/euler.py:140-186, equations/maxwell.py:238-269) and the array
+ * context turns the operator's einsum DAG into device kernels
+ * (arraycontext/dgfem_optimizer.py:181-245).  This library is what a new array
+ * context binds to in place of that code generator.  Each entry point below
+ * names the reference interface it replaces.
+ *
+ * Conventions
+ *  - All functions return 0 (DG_OK) on success or a positive dg_status code;
+ *    dg_last_error_string() gives a message for the calling thread.
+ *  - Field data are device pointers to C-contiguous (K, Np) float64 arrays, one
+ *    per field, element-major: the layout of one meshmode DOFArray group.
+ *    Field order: wave (u, v_x, v_y, v_z); Euler (mass, energy, momentum
+ *    x, y, z); Maxwell (E_x, E_y, E_z, H_x, H_y, H_z).
+ *  - `fields_in` / `fields_out` are HOST arrays of nfields DEVICE pointers.
+ *  - Launches are asynchronous on `stream`; nothing synchronises the host.
+ *  - Hot-path calls allocate nothing.  Plans own their device constants.
+ */
+#ifndef DGFEM_H
+#define DGFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dg_status {
+  DG_OK = 0,
+  DG_ERR_INVALID_ARGUMENT = 1,
+  DG_ERR_UNSUPPORTED = 2,
+  DG_ERR_HIP = 3,
+  DG_ERR_OUT_OF_MEMORY = 4
+} dg_status;
+
+typedef enum dg_equation {
+  DG_EQ_WAVE = 0,     /* equations/wave.py:121-200 */
+  DG_EQ_EULER = 1,    /* equations/euler.py:168-179 (EulerOperator, LF, walls) */
+  DG_EQ_MAXWELL = 2   /* equations/maxwell.py:252-265 (MaxwellOperator, PEC) */
+} dg_equation;
+
+typedef enum dg_precision {
+  DG_PREC_FP64 = 0    /* the drop-in form */
+} dg_precision;
+
+/* Everything a plan needs; all pointers are HOST memory and are copied. */
+typedef struct dg_plan_desc {
+  int32_t equation;          /* dg_equation */
+  int32_t order;             /* polynomial degree, 1..4 */
+  int64_t nelements;         /* K */
+  /* (3, Np, Np) row-major [r][i][j]: M^{-1} S_r^T, the weak derivative with
+   * the inverse mass fused (mass_inverse_fuser.py:39-74) */
+  const double* d_weak;
+  /* (3, Np, Np): V_r V^{-1}, the strong reference derivative */
+  const double* d_strong;
+  /* (Np, 4*Nfp): M^{-1} E, face mass fused with inverse mass
+   * (mass_inverse_fuser.py:21-38) */
+  const double* lift;
+  /* (4, Nfp): volume node index of each face node */
+  const int32_t* face_nodes;
+  /* (K, 25): rx[x][r] = dr_r/dx_x (9 values), then for faces f=0..3:
+   * n_x, n_y, n_z, Fscale = sJ/J */
+  const double* geometry;
+  /* (K, 4*Nfp): flat index e2*Np + i2 of the exterior node of every face
+   * node; boundary face nodes hold -(e*Np + i) - 1 (their own node, negated) */
+  const int32_t* ext_map;
+  /* wave: params[0] = c; Euler: params[0] = gamma; Maxwell: params[0] =
+   * flux_type alpha (0 central .. 1 upwind), with epsilon = mu = 1 as in
+   * equations/maxwell.py:252-254 */
+  double params[4];
+} dg_plan_desc;
+
+typedef struct dg_plan dg_plan;
+
+/* Builds a plan: copies operators, geometry and the exterior map to the
+ * current device.  Replaces the reference's compile-time constant folding of
+ * geometry and gather indices (arraycontext/constants_folder.py:39-115) and
+ * the transpose of LIFT/D (transpose_consts_in_einsums.py:5-84). */
+int dg_plan_create(const dg_plan_desc* desc, dg_plan** plan_out);
+
+/* Frees a plan's device memory.  NULL is accepted. */
+int dg_plan_destroy(dg_plan* plan);
+
+/* Basic facts about a plan: number of fields, Np, Nfp, K. */
+int dg_plan_info(const dg_plan* plan, int32_t* nfields, int32_t* np,
+                 int32_t* nfp, int64_t* nelements);
+
+/* One right-hand-side evaluation, rhs = L(q): the call `compiled_rhs(*args)`
+ * (run/__main__.py:76,109,126) of an array context, i.e. wave_operator
+ * (equations/wave.py:121-200), EulerOperator.operator (equations/euler.py:
+ * 176-179) or MaxwellOperator.operator (equations/maxwell.py:264-265). */
+int dg_rhs(const dg_plan* plan, int32_t precision,
+           const double* const* fields_in, double* const* fields_out,
+           void* stream);
+
+/* One stage of the five-stage, fourth-order 2N-storage Runge-Kutta scheme
+ * (Carpenter & Kennedy 1994), as in libparanumal's LSERK4 that the reference
+ * compares against (assets/h200nvl.patch:42):
+ *   res = a*res + dt*rhs;  q = q + b*res   for every field.
+ * The reference has no time stepper of its own. */
+int dg_lserk_stage(const dg_plan* plan, double a, double b, double dt,
+                   double* const* q, double* const* res,
+                   const double* const* rhs, void* stream);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* dg_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGFEM_H */

@@ -1,0 +1,49 @@
+"""Writes tests/golden/*.npz: small input/output vectors of the oracle.
+
+Each fixture stores the mesh (vertices, element-vertex table), the reference
+node set, the initial state and the oracle's RHS, so a change in the engine's
+mesh or node conventions is detected as well as a change in values.
+
+    python tests/golden/make_golden.py
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from actx_dgfem_suite_amd.workloads import WORKLOADS, initial_state  # noqa: E402
+from actx_dgfem_suite_amd.discretization import make_box_discretization  # noqa: E402
+from oracle import dg_oracle  # noqa: E402
+from tests._helpers import oracle_rhs, random_state  # noqa: E402
+
+FIXTURES = {
+    # the tiny_wave P1 workload: [-1,1]^3, 4^3 x 6 tets, Gaussian sigma 0.25
+    "tiny_wave_p1": ("wave", 1, 4, 1.0, "workload"),
+    "euler_p2_n2": ("euler", 2, 2, 1.4, "random"),
+    "maxwell_p3_n2": ("maxwell", 3, 2, 0.5, "random"),
+    "wave_p4_n2": ("wave", 4, 2, 1.0, "random"),
+}
+
+
+def make(name):
+    equation, order, nel, param, kind = FIXTURES[name]
+    d = make_box_discretization(order, nel)
+    od = dg_oracle.build_discretization(d.mesh.vertices, d.mesh.elements, d.ref.nodes)
+    if kind == "workload":
+        q = initial_state(WORKLOADS[name], d)
+    else:
+        q = random_state(equation, od.x, seed=order)
+    rhs = oracle_rhs(equation, od, q, param)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), equation=equation, order=order,
+                        nel_1d=nel, param=param, vertices=d.mesh.vertices,
+                        elements=d.mesh.elements, ref_nodes=d.ref.nodes, q=q, rhs=rhs)
+
+
+if __name__ == "__main__":
+    for n in FIXTURES:
+        make(n)
+        print("wrote", n)
