@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_HERE, "libdgfem.so")
 
 DG_EQ_WAVE, DG_EQ_EULER, DG_EQ_MAXWELL = 0, 1, 2
 DG_PREC_FP64 = 0
+DG_KERNEL_AUTO, DG_KERNEL_MFMA, DG_KERNEL_LDS = 0, 1, 2
+KERNEL_VARIANTS = {"auto": DG_KERNEL_AUTO, "mfma": DG_KERNEL_MFMA, "lds": DG_KERNEL_LDS}
 
 EQUATION_IDS = {"wave": DG_EQ_WAVE, "euler": DG_EQ_EULER, "maxwell": DG_EQ_MAXWELL}
 PRECISION_IDS = {"fp64": DG_PREC_FP64}
@@ -42,6 +44,8 @@ class DgPlanDesc(ctypes.Structure):
         ("geometry", ctypes.POINTER(ctypes.c_double)),
         ("ext_map", ctypes.POINTER(ctypes.c_int32)),
         ("params", ctypes.c_double * 4),
+        ("variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

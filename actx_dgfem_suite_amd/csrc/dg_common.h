@@ -1,0 +1,161 @@
+// Shared definitions of the DG right-hand-side kernels: sizes, argument
+// blocks and the numerical-flux functors of the three operators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dgfem.h"
+
+namespace dg {
+
+constexpr int kFaces = 4;
+constexpr int kGeo = 25;      // rx (9) + 4 faces x (nx, ny, nz, fscale)
+constexpr int kBlock = 256;
+constexpr int kMaxFields = 6;
+
+constexpr int np_of(int order) { return (order + 1) * (order + 2) * (order + 3) / 6; }
+constexpr int nfp_of(int order) { return (order + 1) * (order + 2) / 2; }
+constexpr int nfields_of(int eq) { return eq == DG_EQ_WAVE ? 4 : (eq == DG_EQ_EULER ? 5 : 6); }
+
+struct InPtrs { const double* p[kMaxFields]; };
+struct OutPtrs { double* p[kMaxFields]; };
+
+struct RhsArgs {
+  InPtrs q;
+  OutPtrs rhs;
+  const double* geo;      // (K, 25)
+  const int32_t* emap;    // (K, 4*Nfp)
+  const double* dmat_t;   // (3, Np(j), Np(i)): transposed weak or strong D
+  const double* lift_t;   // (4*Nfp, Np(i))
+  const int32_t* fmask;   // (4*Nfp)
+  const double* op_tiles;    // MFMA A tiles of D: [r][mblock][kstep][64]
+  const double* lift_tiles;  // MFMA A tiles of LIFT: [mblock][kstep][64]
+  int64_t nelem;
+  double p0;
+};
+
+// XCD-aware work split: blocks are dealt round-robin to the 8 XCDs, so the
+// logical block id gives each XCD one contiguous run of work (neighbour
+// gathers then stay in that XCD's L2).  Needs gridDim.x % 8 == 0.
+__device__ inline int logical_block() {
+  const int xcd_blocks = gridDim.x / 8;
+  return (blockIdx.x % 8) * xcd_blocks + blockIdx.x / 8;
+}
+
+// {{{ numerical fluxes: out[c] is the integrand of +face_mass for field c
+
+// wave_flux, equations/wave.py:80-114: c({v}.n + [u]/2, ({u} + [v].n/2) n),
+// with [a] = exterior - interior; boundary state u+ = -u-, v+ = v- (:152-156).
+struct WaveFlux {
+  static constexpr int NF = 4;
+  __device__ static void boundary(const double* qm, const double* /*n*/, double* qp) {
+    qp[0] = -qm[0];
+    qp[1] = qm[1];
+    qp[2] = qm[2];
+    qp[3] = qm[3];
+  }
+  __device__ static void flux(const double* qm, const double* qp, const double* n,
+                              double c, double* out) {
+    const double uavg = 0.5 * (qm[0] + qp[0]);
+    const double udiff = qp[0] - qm[0];
+    const double vavg_n = 0.5 * ((qm[1] + qp[1]) * n[0] + (qm[2] + qp[2]) * n[1] +
+                                 (qm[3] + qp[3]) * n[2]);
+    const double vjump = (qp[1] - qm[1]) * n[0] + (qp[2] - qm[2]) * n[1] +
+                         (qp[3] - qm[3]) * n[2];
+    out[0] = c * (vavg_n + 0.5 * udiff);
+    const double s = c * (uavg + 0.5 * vjump);
+    out[1] = s * n[0];
+    out[2] = s * n[1];
+    out[3] = s * n[2];
+  }
+};
+
+// Euler, local Lax-Friedrichs: F* = {F}.n - lam/2 (q+ - q-), lam = max(|u|+c)
+// over both sides; wall state reflects momentum.  The operator subtracts
+// face_mass(F*), so out = -F*.  Field order: mass, energy, momentum[3].
+struct EulerFlux {
+  static constexpr int NF = 5;
+  __device__ static void boundary(const double* qm, const double* n, double* qp) {
+    const double mn = qm[2] * n[0] + qm[3] * n[1] + qm[4] * n[2];
+    qp[0] = qm[0];
+    qp[1] = qm[1];
+    qp[2] = qm[2] - 2.0 * mn * n[0];
+    qp[3] = qm[3] - 2.0 * mn * n[1];
+    qp[4] = qm[4] - 2.0 * mn * n[2];
+  }
+  // normal flux F(q).n and wavespeed |u| + sqrt(gamma p / rho)
+  __device__ static void normal_flux(const double* q, const double* n, double gamma,
+                                     double* fn, double* lam) {
+    const double rho = q[0], e = q[1];
+    const double ux = q[2] / rho, uy = q[3] / rho, uz = q[4] / rho;
+    const double p = (gamma - 1.0) * (e - 0.5 * (q[2] * ux + q[3] * uy + q[4] * uz));
+    const double un = ux * n[0] + uy * n[1] + uz * n[2];
+    fn[0] = q[2] * n[0] + q[3] * n[1] + q[4] * n[2];
+    fn[1] = (e + p) * un;
+    fn[2] = q[2] * un + p * n[0];
+    fn[3] = q[3] * un + p * n[1];
+    fn[4] = q[4] * un + p * n[2];
+    *lam = sqrt(ux * ux + uy * uy + uz * uz) + sqrt(gamma * (p / rho));
+  }
+  __device__ static void flux(const double* qm, const double* qp, const double* n,
+                              double gamma, double* out) {
+    double fm[5], fp[5], lm, lp;
+    normal_flux(qm, n, gamma, fm, &lm);
+    normal_flux(qp, n, gamma, fp, &lp);
+    const double lam = fmax(lm, lp);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      out[c] = -(0.5 * (fm[c] + fp[c]) - 0.5 * lam * (qp[c] - qm[c]));
+  }
+};
+
+// Maxwell (eps = mu = 1), flux_type alpha (Hesthaven & Warburton p. 433):
+//   flux_E = -1/2 n x [H] + alpha/2 n x (n x [E])
+//   flux_H =  1/2 n x [E] + alpha/2 n x (n x [H])
+// PEC walls: E+ = -E-, H+ = H-.  The operator subtracts M^{-1} face_mass(flux).
+struct MaxwellFlux {
+  static constexpr int NF = 6;
+  __device__ static void boundary(const double* qm, const double* /*n*/, double* qp) {
+    qp[0] = -qm[0];
+    qp[1] = -qm[1];
+    qp[2] = -qm[2];
+    qp[3] = qm[3];
+    qp[4] = qm[4];
+    qp[5] = qm[5];
+  }
+  __device__ static void cross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  }
+  __device__ static void flux(const double* qm, const double* qp, const double* n,
+                              double alpha, double* out) {
+    double de[3], dh[3], nde[3], ndh[3], nnde[3], nndh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      de[k] = qp[k] - qm[k];
+      dh[k] = qp[3 + k] - qm[3 + k];
+    }
+    cross(n, de, nde);
+    cross(n, dh, ndh);
+    cross(n, nde, nnde);
+    cross(n, ndh, nndh);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      out[k] = -(-0.5 * ndh[k] + 0.5 * alpha * nnde[k]);
+      out[3 + k] = -(0.5 * nde[k] + 0.5 * alpha * nndh[k]);
+    }
+  }
+};
+
+template <int EQ> struct FluxOf;
+template <> struct FluxOf<DG_EQ_WAVE> { using type = WaveFlux; };
+template <> struct FluxOf<DG_EQ_EULER> { using type = EulerFlux; };
+template <> struct FluxOf<DG_EQ_MAXWELL> { using type = MaxwellFlux; };
+
+// }}}
+
+
+}  // namespace dg

@@ -1,0 +1,316 @@
+// Fused FP64 RHS kernel on the matrix cores (v_mfma_f64_16x16x4f64).
+//
+// Each wave owns tiles of 16 consecutive elements, the N dimension of the
+// MFMA.  The constant operators are the A operand, pre-tiled on the host and
+// held in LDS for the whole (persistent) block.  The element data are the B
+// operand, loaded straight from HBM into registers.
+//
+// Operand layouts of v_mfma_f64_16x16x4f64 (lane l, g = l / 16, n = l % 16):
+//   A (16 x 4):  lane l holds A[m = n][k = g]
+//   B (4 x 16):  lane l holds B[k = g][col = n]
+//   C (16 x 16): lane l, register v holds C[row = g + 4 v][col = n]
+// (measured on gfx950: the f64 output interleaves rows across lane groups,
+// unlike the f32 16x16x4 form whose rows are 4 g + v)
+// The contraction index k = g at k-step ks stands for node j = g * KS + ks,
+// i.e. lane group g walks a contiguous run of each element's nodes, so B
+// operands are contiguous loads.  For LIFT the same rule maps lane group g to
+// face g (the 4*Nfp face nodes are face-major), so a lane keeps its face's
+// normal and Fscale in registers.  The host writes A tiles in the same order:
+//   tile[(r * MB + b) * KS + ks][l] = D_r[16 b + n][g * KS + ks]   (0-padded)
+//
+// Element n of the tile is fixed per lane, so per-element geometric factors
+// (rx, normals, Fscale) are applied in registers to MFMA outputs and inputs:
+//   gradient form  t_r = D_r u,           out_x = sum_r rx[x][r] t_r
+//   divergence     out = sum_r D_r w_r,   w_r = sum_x rx[x][r] F_x
+// The two share every A tile.
+#pragma once
+
+#include "dg_common.h"
+
+namespace dg {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f64x4 mfma_f64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int ORDER>
+struct MfmaShape {
+  static constexpr int NP = np_of(ORDER);
+  static constexpr int NFP = nfp_of(ORDER);
+  static constexpr int NFN = kFaces * NFP;
+  static constexpr int MB = (NP + 15) / 16;   // 16-row blocks of the output
+  static constexpr int KS = (NP + 3) / 4;     // k-steps of one D_r contraction
+  static constexpr int KSL = NFP;             // k-steps of the LIFT contraction
+  static constexpr int OP_TILES = 3 * MB * KS;
+  static constexpr int LIFT_TILES = MB * KSL;
+};
+
+// LIFT k-steps whose flux one wave stages in LDS at a time.
+template <int EQ, int ORDER>
+constexpr int flux_chunk() {
+  constexpr int nfp = nfp_of(ORDER);
+  if (nfp <= 6) return nfp;                  // P1, P2: one chunk
+  return EQ == DG_EQ_MAXWELL ? 4 : 5;        // P3 (10 nodes), P4 (15 nodes)
+}
+
+// Levi-Civita sign of (o, x, a) for the curl: (curl F)_o = eps_{o x a} d_x F_a.
+__device__ constexpr int levi(int o, int x, int a) {
+  return (o == x || x == a || o == a) ? 0
+         : (((x - o + 3) % 3 == 1) ? 1 : -1);
+}
+
+// Unroll depth of the volume k-step loops: enough independent MFMAs and loads
+// in flight without hoisting every load of the tile into registers.
+#define KUNROLL 3
+
+template <int EQ, int ORDER>
+__global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
+  using S = MfmaShape<ORDER>;
+  using Flux = typename FluxOf<EQ>::type;
+  constexpr int NP = S::NP, NFP = S::NFP, NFN = S::NFN, MB = S::MB, KS = S::KS;
+  constexpr int NF = nfields_of(EQ);
+  constexpr int WAVES = kBlock / 64;
+
+  constexpr int C = flux_chunk<EQ, ORDER>();
+  constexpr int NCHUNK = (NFP + C - 1) / C;
+
+  __shared__ double s_op[S::OP_TILES * 64];
+  __shared__ double s_lift[S::LIFT_TILES * 64];
+  __shared__ int s_fmask[NFN];
+  // scaled flux of C LIFT k-steps, in MFMA B order: [kk][field][lane]
+  __shared__ double s_flux[WAVES][C * NF * 64];
+
+  const int tid = threadIdx.x;
+  for (int k = tid; k < S::OP_TILES * 64; k += kBlock) s_op[k] = a.op_tiles[k];
+  for (int k = tid; k < S::LIFT_TILES * 64; k += kBlock) s_lift[k] = a.lift_tiles[k];
+  for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
+  __syncthreads();
+
+  const int lane = tid & 63;
+  const int g = lane >> 4;
+  const int n = lane & 15;
+  const int wave = tid >> 6;
+  // batches of WAVES tiles; the batch loop is block-uniform (it holds barriers)
+  const int64_t ntiles = (a.nelem + 15) / 16;
+  const int64_t nbatch = (ntiles + WAVES - 1) / WAVES;
+  const int64_t per = (nbatch + gridDim.x - 1) / gridDim.x;
+  const int64_t b_begin = (int64_t)logical_block() * per;
+  const int64_t b_end = b_begin + per < nbatch ? b_begin + per : nbatch;
+  const double c0 = a.p0;
+  double* flux_buf = s_flux[wave];
+
+  for (int64_t batch = b_begin; batch < b_end; ++batch) {
+    const int64_t tile = batch * WAVES + wave;
+    const int64_t gel = tile * 16 + n;
+    const bool valid = gel < a.nelem;
+    const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
+    const double* geo = a.geo + el * kGeo;
+    double rx[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rx[k] = geo[k];
+
+    f64x4 acc[NF][MB];
+#pragma unroll
+    for (int c = 0; c < NF; ++c)
+#pragma unroll
+      for (int b = 0; b < MB; ++b) acc[c][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    // {{{ volume terms
+
+    if constexpr (EQ == DG_EQ_WAVE) {
+      // u_t = -c sum_r D_r w_r (w_r = sum_x rx[x][r] v_x); v_x,t = -c sum_r rx[x][r] D_r u
+      f64x4 t[3][MB];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) t[r][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll KUNROLL
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = g * KS + ks;
+        double u = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+        if (j < NP) {
+          const int64_t idx = el * NP + j;
+          u = a.q.p[0][idx];
+          vx = a.q.p[1][idx];
+          vy = a.q.p[2][idx];
+          vz = a.q.p[3][idx];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const double w = rx[r] * vx + rx[3 + r] * vy + rx[6 + r] * vz;
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const double av = s_op[((r * MB + b) * KS + ks) * 64 + lane];
+            t[r][b] = mfma_f64(av, u, t[r][b]);
+            acc[0][b] = mfma_f64(av, w, acc[0][b]);
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < MB; ++b) {
+        acc[0][b] *= -c0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+          acc[1 + x][b] = -c0 * (rx[3 * x] * t[0][b] + rx[3 * x + 1] * t[1][b] +
+                                 rx[3 * x + 2] * t[2][b]);
+      }
+    } else if constexpr (EQ == DG_EQ_EULER) {
+      // q_t = sum_r D_r w_{c,r},  w_{c,r} = sum_x rx[x][r] F_{c,x}(q)
+      const double gamma = c0;
+#pragma unroll KUNROLL
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = g * KS + ks;
+        double w[5][3];
+        if (j < NP) {
+          const int64_t idx = el * NP + j;
+          const double rho = a.q.p[0][idx], e = a.q.p[1][idx];
+          const double m[3] = {a.q.p[2][idx], a.q.p[3][idx], a.q.p[4][idx]};
+          const double inv_rho = 1.0 / rho;
+          const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
+          const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
+          const double h = (e + p) * inv_rho;
+          double f[5][3];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            f[0][x] = m[x];
+            f[1][x] = m[x] * h;
+#pragma unroll
+            for (int y = 0; y < 3; ++y) f[2 + y][x] = m[y] * u[x] + (x == y ? p : 0.0);
+          }
+#pragma unroll
+          for (int c = 0; c < 5; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              w[c][r] = rx[r] * f[c][0] + rx[3 + r] * f[c][1] + rx[6 + r] * f[c][2];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) w[c][0] = w[c][1] = w[c][2] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const double av = s_op[((r * MB + b) * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) acc[c][b] = mfma_f64(av, w[c][r], acc[c][b]);
+          }
+      }
+    } else {
+      // E_t = curl H, H_t = -curl E, strong form: d_x F = sum_r rx[x][r] D_r F
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        f64x4 t[3][MB];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < MB; ++b) t[r][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll KUNROLL
+        for (int ks = 0; ks < KS; ++ks) {
+          const int j = g * KS + ks;
+          const double u = j < NP ? a.q.p[c][el * NP + j] : 0.0;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int b = 0; b < MB; ++b)
+              t[r][b] = mfma_f64(s_op[((r * MB + b) * KS + ks) * 64 + lane], u, t[r][b]);
+        }
+        const int comp = c % 3;
+        const bool is_h = c >= 3;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+#pragma unroll
+          for (int o = 0; o < 3; ++o) {
+            const int s = levi(o, x, comp);
+            if (s == 0) continue;
+            // E fields feed -curl E into H rhs; H fields feed curl H into E rhs
+            const double sign = is_h ? (double)s : -(double)s;
+            const int out = is_h ? o : 3 + o;
+#pragma unroll
+            for (int b = 0; b < MB; ++b)
+              acc[out][b] += sign * (rx[3 * x] * t[0][b] + rx[3 * x + 1] * t[1][b] +
+                                     rx[3 * x + 2] * t[2][b]);
+          }
+        }
+      }
+    }
+
+    // }}}
+
+    // {{{ surface terms
+    //
+    // LIFT k-step ks, lane group g consumes the flux at node ks of face g of
+    // element n.  The flux is produced in a gather-friendly order instead --
+    // consecutive lanes take consecutive nodes of one face, so the exterior
+    // gathers of neighbouring lanes hit the same neighbour element -- and
+    // handed over through LDS, C k-steps at a time.
+
+#pragma unroll 1
+    for (int ch = 0; ch < NCHUNK; ++ch) {
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < C; ++it) {
+        const int item = it * 64 + lane;          // (face pair p, node kk)
+        const int kk = item % C;
+        const int p = item / C;                   // 0..63: p = fg * 16 + fn_el
+        const int fg = p >> 4, fe = p & 15;
+        const int ks = ch * C + kk;
+        if (ks < NFP) {
+          const int64_t pg = tile * 16 + fe;
+          const int64_t pel = pg < a.nelem ? pg : a.nelem - 1;
+          const double* pgeo = a.geo + pel * kGeo + 9 + 4 * fg;
+          const double nrm[3] = {pgeo[0], pgeo[1], pgeo[2]};
+          const double fscale = pgeo[3];
+          const int fn = fg * NFP + ks;
+          const int vn = s_fmask[fn];
+          double qm[NF], qp[NF], out[NF];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][pel * NP + vn];
+          const int32_t ext = a.emap[pel * NFN + fn];
+          if (ext >= 0) {
+#pragma unroll
+            for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
+          } else {
+            Flux::boundary(qm, nrm, qp);
+          }
+          Flux::flux(qm, qp, nrm, c0, out);
+#pragma unroll
+          for (int c = 0; c < NF; ++c) flux_buf[(kk * NF + c) * 64 + p] = fscale * out[c];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < C; ++kk) {
+        const int ks = ch * C + kk;
+        if (ks < NFP) {
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const double av = s_lift[(b * NFP + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < NF; ++c)
+              acc[c][b] = mfma_f64(av, flux_buf[(kk * NF + c) * 64 + lane], acc[c][b]);
+          }
+        }
+      }
+    }
+
+    // }}}
+
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        double* dst = a.rhs.p[c] + gel * NP;
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * b + g + 4 * v;
+            if (i < NP) dst[i] = acc[c][b][v];
+          }
+      }
+    }
+  }
+}
+
+}  // namespace dg

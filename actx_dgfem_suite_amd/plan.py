@@ -28,13 +28,17 @@ class DGPlan:
 
     ``param`` is the wave speed c (wave), gamma (Euler) or the flux_type alpha
     (Maxwell), matching equations/wave.py:279, equations/euler.py:151 and
-    equations/maxwell.py:252-254.
+    equations/maxwell.py:252-254.  ``variant`` picks the FP64 kernel:
+    ``"auto"``/``"mfma"`` (matrix cores) or ``"lds"`` (the baseline).
     """
 
     def __init__(self, discr: Discretization, equation: str, param: float,
-                 device: torch.device | int | None = None):
+                 device: torch.device | int | None = None, variant: str = "auto"):
         if equation not in _lib.EQUATION_IDS:
             raise NotImplementedError(f"equation '{equation}' not supported")
+        if variant not in _lib.KERNEL_VARIANTS:
+            raise ValueError(f"kernel variant '{variant}' (known: {sorted(_lib.KERNEL_VARIANTS)})")
+        self.variant = variant
         if not torch.cuda.is_available():
             raise RuntimeError("DGPlan needs a ROCm GPU (no CPU fallback)")
         self.lib = _lib.load_library()
@@ -65,6 +69,7 @@ class DGPlan:
         desc.geometry = _dptr(h["geometry"], ctypes.c_double)
         desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
         desc.params[0] = self.param
+        desc.variant = _lib.KERNEL_VARIANTS[variant]
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib, self.lib.dg_plan_create(ctypes.byref(desc),

@@ -35,6 +35,7 @@ def _parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default="wave_p3")
     p.add_argument("--precision", default="fp64")
+    p.add_argument("--variant", default="auto", help="FP64 kernel: auto | mfma | lds")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate CPU time budget of the cpu_baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -113,7 +114,7 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     d = w.discretization()
-    plan = DGPlan(d, w.equation, w.param, device=dev)
+    plan = DGPlan(d, w.equation, w.param, device=dev, variant=args.variant)
     q = [torch.from_numpy(np.ascontiguousarray(f)).to(dev) for f in initial_state(w, d)]
     out = plan.empty_fields()
     nf = len(q)
@@ -176,6 +177,7 @@ def main():
             "dofs_per_field": k * npn,
             "fields": nf,
             "precision": args.precision,
+            "variant": args.variant,
         },
         "gflops": f_alg / t_kernel / 1e9,
         "gdofs_per_s": k * npn / t_kernel / 1e9,
@@ -186,7 +188,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"rhs_fp64_kernel<{w.equation},P{w.order}>",
+            "kernel": f"rhs_{'lds' if args.variant == 'lds' else 'mfma'}_kernel<{w.equation},P{w.order}>",
             "kernel_us": kernel_ms * 1e3,
             "algorithmic_bytes": b_alg,
             "algorithmic_flops": f_alg,

@@ -45,6 +45,12 @@ typedef enum dg_equation {
   DG_EQ_MAXWELL = 2   /* equations/maxwell.py:252-265 (MaxwellOperator, PEC) */
 } dg_equation;
 
+typedef enum dg_kernel_variant {
+  DG_KERNEL_AUTO = 0,   /* the fastest variant (currently MFMA) */
+  DG_KERNEL_MFMA = 1,   /* fused kernel on v_mfma_f64_16x16x4f64 */
+  DG_KERNEL_LDS = 2     /* one thread per node, LDS-staged (baseline) */
+} dg_kernel_variant;
+
 typedef enum dg_precision {
   DG_PREC_FP64 = 0    /* the drop-in form */
 } dg_precision;
@@ -74,6 +80,8 @@ typedef struct dg_plan_desc {
    * flux_type alpha (0 central .. 1 upwind), with epsilon = mu = 1 as in
    * equations/maxwell.py:252-254 */
   double params[4];
+  int32_t variant;           /* dg_kernel_variant */
+  int32_t reserved;          /* must be 0 */
 } dg_plan_desc;
 
 typedef struct dg_plan dg_plan;

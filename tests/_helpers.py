@@ -33,6 +33,18 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
+def field_errors(got, ref) -> list[float]:
+    """Relative L2 error of every field.  A field whose reference RHS is
+    negligible (below 1e-3 of the whole state's RHS, e.g. an exact eigenmode)
+    is measured against the whole state's scale instead of its own round-off."""
+    total = np.linalg.norm(np.ravel(ref))
+    out = []
+    for c in range(len(ref)):
+        den = max(np.linalg.norm(np.ravel(ref[c])), 1e-3 * total, 1e-300)
+        out.append(float(np.linalg.norm(np.ravel(got[c]) - np.ravel(ref[c])) / den))
+    return out
+
+
 def interior_elements(od) -> np.ndarray:
     """Elements none of whose faces lies on the boundary."""
     return np.nonzero(np.all(od.ext_index[:, :, 0] >= 0, axis=1))[0]
