@@ -75,28 +75,45 @@ __global__ __launch_bounds__(kBlock) void lserk_stage_kernel(LserkArgs s) {
 
 using RhsKernel = void (*)(RhsArgs);
 
-#define DG_DEFINE_PICK(NAME, KERNEL)                                         \
+// A kernel and its launch shape: threads per block and elements per batch.
+struct KernelInfo {
+  RhsKernel fn = nullptr;
+  int threads = 0;
+  int batch_elems = 0;
+};
+
+template <int EQ, int ORDER>
+KernelInfo lds_info() {
+  return {rhs_lds_kernel<EQ, ORDER>, kBlock, kBlock / np_of(ORDER)};
+}
+
+template <int EQ, int ORDER>
+KernelInfo mfma_info() {
+  return {rhs_mfma_kernel<EQ, ORDER>, kBlock, 16 * (kBlock / 64)};
+}
+
+#define DG_DEFINE_PICK(NAME, INFO)                                           \
   template <int EQ>                                                          \
-  RhsKernel NAME##_order(int order) {                                        \
+  KernelInfo NAME##_order(int order) {                                       \
     switch (order) {                                                         \
-      case 1: return KERNEL<EQ, 1>;                                          \
-      case 2: return KERNEL<EQ, 2>;                                          \
-      case 3: return KERNEL<EQ, 3>;                                          \
-      case 4: return KERNEL<EQ, 4>;                                          \
-      default: return nullptr;                                               \
+      case 1: return INFO<EQ, 1>();                                          \
+      case 2: return INFO<EQ, 2>();                                          \
+      case 3: return INFO<EQ, 3>();                                          \
+      case 4: return INFO<EQ, 4>();                                          \
+      default: return KernelInfo{};                                          \
     }                                                                        \
   }                                                                          \
-  RhsKernel NAME(int eq, int order) {                                        \
+  KernelInfo NAME(int eq, int order) {                                       \
     switch (eq) {                                                            \
       case DG_EQ_WAVE: return NAME##_order<DG_EQ_WAVE>(order);               \
       case DG_EQ_EULER: return NAME##_order<DG_EQ_EULER>(order);             \
       case DG_EQ_MAXWELL: return NAME##_order<DG_EQ_MAXWELL>(order);         \
-      default: return nullptr;                                               \
+      default: return KernelInfo{};                                          \
     }                                                                        \
   }
 
-DG_DEFINE_PICK(pick_lds, rhs_lds_kernel)
-DG_DEFINE_PICK(pick_mfma, rhs_mfma_kernel)
+DG_DEFINE_PICK(pick_lds, lds_info)
+DG_DEFINE_PICK(pick_mfma, mfma_info)
 
 // A-operand tiles of the MFMA kernel (see rhs_mfma.h for the layout).
 void build_tiles(int np, int nfp, const double* op, const double* lift,
@@ -133,7 +150,7 @@ struct dg_plan {
   int device = 0;
   int grid = 0;          // persistent grid of the RHS kernel
   int lserk_grid = 0;
-  RhsKernel kernel = nullptr;
+  KernelInfo kernel;
   int variant = 0;
   double* geo = nullptr;
   int32_t* emap = nullptr;
@@ -258,13 +275,12 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     return fail(DG_ERR_HIP, "hipGetDeviceProperties failed");
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)p->kernel, kBlock, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)p->kernel.fn,
+                                                    p->kernel.threads, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
-  // work items: element batches (LDS variant) or 16-element tiles per wave
-  const int64_t nbatch = p->variant == DG_KERNEL_LDS ? (k + kBlock / np - 1) / (kBlock / np)
-                                                     : ((k + 15) / 16 + 3) / 4;
+  const int64_t nbatch = (k + p->kernel.batch_elems - 1) / p->kernel.batch_elems;
   int64_t grid = (int64_t)prop.multiProcessorCount * per_cu;
   if (grid > nbatch) grid = nbatch;
   // the kernel's XCD remap needs a multiple of 8 blocks; spare blocks idle
@@ -312,7 +328,8 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
   a.lift_tiles = p->lift_tiles;
   a.nelem = p->nelem;
   a.p0 = p->params[0];
-  hipLaunchKernelGGL(p->kernel, dim3(p->grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(p->kernel.fn, dim3(p->grid), dim3(p->kernel.threads), 0,
+                     (hipStream_t)stream, a);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
 }

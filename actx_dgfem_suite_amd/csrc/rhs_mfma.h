@@ -1,25 +1,36 @@
 // Fused FP64 RHS kernel on the matrix cores (v_mfma_f64_16x16x4f64).
 //
-// Each wave owns tiles of 16 consecutive elements, the N dimension of the
-// MFMA.  The constant operators are the A operand, pre-tiled on the host and
-// held in LDS for the whole (persistent) block.  The element data are the B
-// operand, loaded straight from HBM into registers.
+// Each wave owns a contiguous run of tiles of 16 consecutive elements (the N
+// dimension of the MFMA) and works on them independently of the other waves
+// of its block -- the only block-wide barrier is the initial load of the
+// constant operators into LDS -- so the memory, LDS, VALU and MFMA phases of
+// different waves overlap freely.
 //
-// Operand layouts of v_mfma_f64_16x16x4f64 (lane l, g = l / 16, n = l % 16):
+// Per tile:
+//   volume: B operands straight from HBM into registers (lane group g loads a
+//     contiguous run of each element's nodes), A operands (pre-tiled
+//     operators) from LDS;
+//   surface, C LIFT k-steps at a time: the wave evaluates the numerical flux
+//     in a gather-friendly order (consecutive lanes = consecutive nodes of one
+//     face, so the exterior gathers of neighbouring lanes hit the same
+//     neighbour element), hands it to itself through a small LDS buffer in
+//     MFMA B order and runs those LIFT k-steps;
+//   output: accumulators go through the same buffer to contiguous stores.
+//
+// Operand layouts of v_mfma_f64_16x16x4f64 (lane l, g = l / 16, n = l % 16),
+// measured on gfx950:
 //   A (16 x 4):  lane l holds A[m = n][k = g]
 //   B (4 x 16):  lane l holds B[k = g][col = n]
 //   C (16 x 16): lane l, register v holds C[row = g + 4 v][col = n]
-// (measured on gfx950: the f64 output interleaves rows across lane groups,
-// unlike the f32 16x16x4 form whose rows are 4 g + v)
-// The contraction index k = g at k-step ks stands for node j = g * KS + ks,
-// i.e. lane group g walks a contiguous run of each element's nodes, so B
-// operands are contiguous loads.  For LIFT the same rule maps lane group g to
-// face g (the 4*Nfp face nodes are face-major), so a lane keeps its face's
-// normal and Fscale in registers.  The host writes A tiles in the same order:
+// (the f64 output interleaves rows across lane groups, unlike the f32 16x16x4
+// form whose rows are 4 g + v).  The contraction index k = g at k-step ks
+// stands for node j = g * KS + ks.  For LIFT the same rule maps lane group g
+// to face g (face nodes are face-major).  The host writes A tiles in the same
+// order:
 //   tile[(r * MB + b) * KS + ks][l] = D_r[16 b + n][g * KS + ks]   (0-padded)
 //
 // Element n of the tile is fixed per lane, so per-element geometric factors
-// (rx, normals, Fscale) are applied in registers to MFMA outputs and inputs:
+// are applied in registers to MFMA inputs and outputs:
 //   gradient form  t_r = D_r u,           out_x = sum_r rx[x][r] t_r
 //   divergence     out = sum_r D_r w_r,   w_r = sum_x rx[x][r] F_x
 // The two share every A tile.
@@ -51,9 +62,24 @@ struct MfmaShape {
 template <int EQ, int ORDER>
 constexpr int flux_chunk() {
   constexpr int nfp = nfp_of(ORDER);
-  if (nfp <= 6) return nfp;                  // P1, P2: one chunk
-  return EQ == DG_EQ_MAXWELL ? 4 : 5;        // P3 (10 nodes), P4 (15 nodes)
+  return nfp <= 6 ? nfp : 5;                 // P1, P2: one chunk; P3, P4: 5 nodes
 }
+
+constexpr int kFluxRow = 65;                 // padded flux row (spreads LDS banks)
+
+// Per-wave LDS buffer (doubles): one flux chunk, or one field of the output tile.
+template <int EQ, int ORDER>
+constexpr int wave_buffer() {
+  constexpr int flux = flux_chunk<EQ, ORDER>() * nfields_of(EQ) * kFluxRow;
+  constexpr int out = 16 * np_of(ORDER);
+  return flux > out ? flux : out;
+}
+
+// Orders this wave's LDS accesses across lanes: waits for the wave's
+// outstanding LDS operations and stops the compiler from moving memory
+// accesses across this point.  (A wavefront-scope fence does not order LDS
+// accesses of different lanes against each other.)
+__device__ inline void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 // Levi-Civita sign of (o, x, a) for the curl: (curl F)_o = eps_{o x a} d_x F_a.
 __device__ constexpr int levi(int o, int x, int a) {
@@ -75,12 +101,15 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
 
   constexpr int C = flux_chunk<EQ, ORDER>();
   constexpr int NCHUNK = (NFP + C - 1) / C;
+  constexpr int WB = wave_buffer<EQ, ORDER>();
+  constexpr int FROW = kFluxRow;
 
   __shared__ double s_op[S::OP_TILES * 64];
   __shared__ double s_lift[S::LIFT_TILES * 64];
   __shared__ int s_fmask[NFN];
-  // scaled flux of C LIFT k-steps, in MFMA B order: [kk][field][lane]
-  __shared__ double s_flux[WAVES][C * NF * 64];
+  // per wave: scaled flux of C LIFT k-steps in MFMA B order [kk][field][FROW],
+  // then one field of the output tile
+  __shared__ double s_buf[WAVES][WB];
 
   const int tid = threadIdx.x;
   for (int k = tid; k < S::OP_TILES * 64; k += kBlock) s_op[k] = a.op_tiles[k];
@@ -91,18 +120,17 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   const int lane = tid & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
-  const int wave = tid >> 6;
-  // batches of WAVES tiles; the batch loop is block-uniform (it holds barriers)
+  // contiguous tile runs per wave; waves of a block are independent
   const int64_t ntiles = (a.nelem + 15) / 16;
-  const int64_t nbatch = (ntiles + WAVES - 1) / WAVES;
-  const int64_t per = (nbatch + gridDim.x - 1) / gridDim.x;
-  const int64_t b_begin = (int64_t)logical_block() * per;
-  const int64_t b_end = b_begin + per < nbatch ? b_begin + per : nbatch;
+  const int64_t nwaves = (int64_t)gridDim.x * WAVES;
+  const int64_t wid = (int64_t)logical_block() * WAVES + (tid >> 6);
+  const int64_t per = (ntiles + nwaves - 1) / nwaves;
+  const int64_t t_begin = wid * per;
+  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
   const double c0 = a.p0;
-  double* flux_buf = s_flux[wave];
+  double* const buf = s_buf[tid >> 6];
 
-  for (int64_t batch = b_begin; batch < b_end; ++batch) {
-    const int64_t tile = batch * WAVES + wave;
+  for (int64_t tile = t_begin; tile < t_end; ++tile) {
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
@@ -238,36 +266,30 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
 
     // }}}
 
-    // {{{ surface terms
-    //
-    // LIFT k-step ks, lane group g consumes the flux at node ks of face g of
-    // element n.  The flux is produced in a gather-friendly order instead --
-    // consecutive lanes take consecutive nodes of one face, so the exterior
-    // gathers of neighbouring lanes hit the same neighbour element -- and
-    // handed over through LDS, C k-steps at a time.
+    // {{{ surface terms, C LIFT k-steps per chunk
 
 #pragma unroll 1
     for (int ch = 0; ch < NCHUNK; ++ch) {
-      __syncthreads();
+      // producer: items (face pair p, node kk) with kk fastest
 #pragma unroll
       for (int it = 0; it < C; ++it) {
-        const int item = it * 64 + lane;          // (face pair p, node kk)
+        const int item = it * 64 + lane;
         const int kk = item % C;
-        const int p = item / C;                   // 0..63: p = fg * 16 + fn_el
-        const int fg = p >> 4, fe = p & 15;
+        const int p = item / C;                   // p = face * 16 + element
         const int ks = ch * C + kk;
         if (ks < NFP) {
-          const int64_t pg = tile * 16 + fe;
-          const int64_t pel = pg < a.nelem ? pg : a.nelem - 1;
-          const double* pgeo = a.geo + pel * kGeo + 9 + 4 * fg;
+          const int fg = p >> 4, fe = p & 15;
+          int64_t pe = tile * 16 + fe;
+          pe = pe < a.nelem ? pe : a.nelem - 1;
+          const double* pgeo = a.geo + pe * kGeo + 9 + 4 * fg;
           const double nrm[3] = {pgeo[0], pgeo[1], pgeo[2]};
           const double fscale = pgeo[3];
           const int fn = fg * NFP + ks;
-          const int vn = s_fmask[fn];
+          const int64_t vn = pe * NP + s_fmask[fn];
           double qm[NF], qp[NF], out[NF];
 #pragma unroll
-          for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][pel * NP + vn];
-          const int32_t ext = a.emap[pel * NFN + fn];
+          for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
+          const int32_t ext = a.emap[pe * NFN + fn];
           if (ext >= 0) {
 #pragma unroll
             for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
@@ -276,10 +298,11 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           }
           Flux::flux(qm, qp, nrm, c0, out);
 #pragma unroll
-          for (int c = 0; c < NF; ++c) flux_buf[(kk * NF + c) * 64 + p] = fscale * out[c];
+          for (int c = 0; c < NF; ++c) buf[(kk * NF + c) * FROW + p] = fscale * out[c];
         }
       }
-      __syncthreads();
+      wave_lds_fence();
+      // consumer
 #pragma unroll
       for (int kk = 0; kk < C; ++kk) {
         const int ks = ch * C + kk;
@@ -289,27 +312,34 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
             const double av = s_lift[(b * NFP + ks) * 64 + lane];
 #pragma unroll
             for (int c = 0; c < NF; ++c)
-              acc[c][b] = mfma_f64(av, flux_buf[(kk * NF + c) * 64 + lane], acc[c][b]);
+              acc[c][b] = mfma_f64(av, buf[(kk * NF + c) * FROW + lane], acc[c][b]);
           }
         }
       }
+      wave_lds_fence();
     }
 
     // }}}
 
-    if (valid) {
+    // {{{ output, one field at a time through the buffer to contiguous stores
+
 #pragma unroll
-      for (int c = 0; c < NF; ++c) {
-        double* dst = a.rhs.p[c] + gel * NP;
+    for (int c = 0; c < NF; ++c) {
 #pragma unroll
-        for (int b = 0; b < MB; ++b)
+      for (int b = 0; b < MB; ++b)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 16 * b + g + 4 * v;
-            if (i < NP) dst[i] = acc[c][b][v];
-          }
-      }
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * b + g + 4 * v;
+          if (i < NP) buf[n * NP + i] = acc[c][b][v];
+        }
+      wave_lds_fence();
+      double* dst = a.rhs.p[c] + tile * 16 * NP;
+      for (int idx = lane; idx < 16 * NP; idx += 64)
+        if (tile * 16 + idx / NP < a.nelem) dst[idx] = buf[idx];
+      wave_lds_fence();
     }
+
+    // }}}
   }
 }
 
