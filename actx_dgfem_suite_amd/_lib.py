@@ -14,12 +14,13 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdgfem.so")
 
 DG_EQ_WAVE, DG_EQ_EULER, DG_EQ_MAXWELL = 0, 1, 2
-DG_PREC_FP64 = 0
+DG_PREC_FP64, DG_PREC_FP32, DG_PREC_BF16X3 = 0, 1, 2
 DG_KERNEL_AUTO, DG_KERNEL_MFMA, DG_KERNEL_LDS = 0, 1, 2
 KERNEL_VARIANTS = {"auto": DG_KERNEL_AUTO, "mfma": DG_KERNEL_MFMA, "lds": DG_KERNEL_LDS}
 
 EQUATION_IDS = {"wave": DG_EQ_WAVE, "euler": DG_EQ_EULER, "maxwell": DG_EQ_MAXWELL}
-PRECISION_IDS = {"fp64": DG_PREC_FP64}
+PRECISION_IDS = {"fp64": DG_PREC_FP64, "fp32": DG_PREC_FP32, "bf16x3": DG_PREC_BF16X3}
+MIXED_PRECISIONS = ("fp32", "bf16x3")
 
 # every symbol include/dgfem.h declares
 EXPORTED_SYMBOLS = (
@@ -27,7 +28,9 @@ EXPORTED_SYMBOLS = (
     "dg_plan_destroy",
     "dg_plan_info",
     "dg_rhs",
+    "dg_rhs_mixed",
     "dg_lserk_stage",
+    "dg_lserk_stage_mixed",
     "dg_last_error_string",
 )
 
@@ -74,6 +77,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_lserk_stage.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, _PP, _PP, _PP, ctypes.c_void_p]
     lib.dg_lserk_stage.restype = ctypes.c_int
+    lib.dg_rhs_mixed.argtypes = [ctypes.c_void_p, ctypes.c_int32, _PP, _PP, ctypes.c_void_p]
+    lib.dg_rhs_mixed.restype = ctypes.c_int
+    lib.dg_lserk_stage_mixed.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _PP, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_lserk_stage_mixed.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []
     lib.dg_last_error_string.restype = ctypes.c_char_p
     return lib

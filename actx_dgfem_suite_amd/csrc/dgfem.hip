@@ -26,6 +26,7 @@
 #include "dg_common.h"
 #include "rhs_lds.h"
 #include "rhs_mfma.h"
+#include "rhs_mixed.h"
 
 namespace {
 
@@ -115,6 +116,122 @@ KernelInfo mfma_info() {
 DG_DEFINE_PICK(pick_lds, lds_info)
 DG_DEFINE_PICK(pick_mfma, mfma_info)
 
+using MixedKernel = void (*)(MixedArgs);
+
+template <int EQ, int PREC>
+MixedKernel pick_mixed_order(int order) {
+  switch (order) {
+    case 1: return rhs_mixed_kernel<EQ, PREC, 1>;
+    case 2: return rhs_mixed_kernel<EQ, PREC, 2>;
+    case 3: return rhs_mixed_kernel<EQ, PREC, 3>;
+    case 4: return rhs_mixed_kernel<EQ, PREC, 4>;
+    default: return nullptr;
+  }
+}
+
+template <int PREC>
+MixedKernel pick_mixed_prec(int eq, int order) {
+  switch (eq) {
+    case DG_EQ_WAVE: return pick_mixed_order<DG_EQ_WAVE, PREC>(order);
+    case DG_EQ_EULER: return pick_mixed_order<DG_EQ_EULER, PREC>(order);
+    case DG_EQ_MAXWELL: return pick_mixed_order<DG_EQ_MAXWELL, PREC>(order);
+    default: return nullptr;
+  }
+}
+
+MixedKernel pick_mixed(int prec, int eq, int order) {
+  if (prec == DG_PREC_FP32) return pick_mixed_prec<DG_PREC_FP32>(eq, order);
+  if (prec == DG_PREC_BF16X3) return pick_mixed_prec<DG_PREC_BF16X3>(eq, order);
+  return nullptr;
+}
+
+// A tiles of the mixed kernel (layout in rhs_mixed.h): 32-bit words, either
+// one float per lane (FP32) or four (hi, lo) bf16 pairs packed in 4 words.
+void build_mixed_tiles(int prec, int np, int nfp, const double* op, const double* lift,
+                       std::vector<uint32_t>& op_words, std::vector<uint32_t>& lift_words) {
+  const int kg = prec == DG_PREC_BF16X3 ? 4 : 1;
+  const int aw = prec == DG_PREC_BF16X3 ? 4 : 1;
+  const int mb = (np + 15) / 16;
+  const int ks_n = ((np + 3) / 4 + kg - 1) / kg * kg;
+  const int ksl = (nfp + kg - 1) / kg * kg;
+  const int nfn = kFaces * nfp;
+  auto pack = [&](std::vector<uint32_t>& dst, size_t base, const double* vals) {
+    if (prec == DG_PREC_FP32) {
+      const float f = (float)vals[0];
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      dst[base] = u;
+      return;
+    }
+    uint16_t hi[4], lo[4];
+    for (int i = 0; i < 4; ++i) {
+      const float f = (float)vals[i];
+      hi[i] = bf16_bits(f);
+      lo[i] = bf16_bits(f - bf16_value(hi[i]));
+    }
+    dst[base + 0] = hi[0] | ((uint32_t)hi[1] << 16);
+    dst[base + 1] = hi[2] | ((uint32_t)hi[3] << 16);
+    dst[base + 2] = lo[0] | ((uint32_t)lo[1] << 16);
+    dst[base + 3] = lo[2] | ((uint32_t)lo[3] << 16);
+  };
+  op_words.assign((size_t)3 * mb * (ks_n / kg) * 64 * aw, 0u);
+  lift_words.assign((size_t)mb * (ksl / kg) * 64 * aw, 0u);
+  double vals[4];
+  for (int r = 0; r < 3; ++r)
+    for (int b = 0; b < mb; ++b)
+      for (int kq = 0; kq < ks_n / kg; ++kq)
+        for (int l = 0; l < 64; ++l) {
+          const int m = 16 * b + (l & 15);
+          for (int i = 0; i < kg; ++i) {
+            const int j = (l >> 4) * ks_n + kq * kg + i;
+            vals[i] = (m < np && j < np) ? op[((size_t)r * np + m) * np + j] : 0.0;
+          }
+          pack(op_words, ((((size_t)r * mb + b) * (ks_n / kg) + kq) * 64 + l) * aw, vals);
+        }
+  for (int b = 0; b < mb; ++b)
+    for (int kq = 0; kq < ksl / kg; ++kq)
+      for (int l = 0; l < 64; ++l) {
+        const int m = 16 * b + (l & 15);
+        for (int i = 0; i < kg; ++i) {
+          const int ks = kq * kg + i;
+          vals[i] = (m < np && ks < nfp) ? lift[(size_t)m * nfn + (l >> 4) * nfp + ks] : 0.0;
+        }
+        pack(lift_words, (((size_t)b * (ksl / kg) + kq) * 64 + l) * aw, vals);
+      }
+}
+
+// {{{ mixed-precision LSERK stage: FP64 state and residual, FP32 RHS in,
+// FP32 shadow of the state out
+
+struct LserkMixedArgs {
+  OutPtrs q;
+  OutPtrs res;
+  const float* rhs[kMaxFields];
+  float* q32[kMaxFields];
+  int64_t n;
+  int nf;
+  double a, b, dt;
+};
+
+__global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int f = 0; f < s.nf; ++f) {
+    double* q = s.q.p[f];
+    double* res = s.res.p[f];
+    const float* rhs = s.rhs[f];
+    float* q32 = s.q32[f];
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < s.n; k += stride) {
+      const double r = s.a * res[k] + s.dt * (double)rhs[k];
+      res[k] = r;
+      const double qn = q[k] + s.b * r;
+      q[k] = qn;
+      q32[k] = (float)qn;
+    }
+  }
+}
+
+// }}}
+
 // A-operand tiles of the MFMA kernel (see rhs_mfma.h for the layout).
 void build_tiles(int np, int nfp, const double* op, const double* lift,
                  std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
@@ -159,6 +276,11 @@ struct dg_plan {
   int32_t* fmask = nullptr;
   double* op_tiles = nullptr;
   double* lift_tiles = nullptr;
+  // mixed-precision forms: A tiles and launch grid per precision
+  uint32_t* mixed_op[2] = {nullptr, nullptr};
+  uint32_t* mixed_lift[2] = {nullptr, nullptr};
+  MixedKernel mixed_kernel[2] = {nullptr, nullptr};
+  int mixed_grid[2] = {0, 0};
 };
 
 namespace {
@@ -172,6 +294,10 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->fmask);
   (void)hipFree(p->op_tiles);
   (void)hipFree(p->lift_tiles);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(p->mixed_op[k]);
+    (void)hipFree(p->mixed_lift[k]);
+  }
   delete p;
 }
 
@@ -199,6 +325,63 @@ int check_ptrs(const void* const* ptrs, int n, const char* what) {
 extern "C" {
 
 const char* dg_last_error_string(void) { return g_last_error.c_str(); }
+
+int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, float* const* out,
+                 void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (precision != DG_PREC_FP32 && precision != DG_PREC_BF16X3)
+    return fail(DG_ERR_UNSUPPORTED, "mixed precision " + std::to_string(precision) +
+                                        " not supported (FP32 or BF16X3)");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
+    return rc;
+  const int k = precision == DG_PREC_FP32 ? 0 : 1;
+  MixedArgs a{};
+  for (int c = 0; c < p->nfields; ++c) {
+    a.q[c] = in[c];
+    a.rhs[c] = out[c];
+  }
+  a.geo = p->geo;
+  a.emap = p->emap;
+  a.fmask = p->fmask;
+  a.op_tiles = p->mixed_op[k];
+  a.lift_tiles = p->mixed_lift[k];
+  a.nelem = p->nelem;
+  a.p0 = p->params[0];
+  hipLaunchKernelGGL(p->mixed_kernel[k], dim3(p->mixed_grid[k]), dim3(kBlock), 0,
+                     (hipStream_t)stream, a);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
+int dg_lserk_stage_mixed(const dg_plan* p, double a, double b, double dt, double* const* q,
+                         double* const* res, const float* const* rhs, float* const* q32,
+                         void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)q, p->nfields, "q")) ||
+      (rc = check_ptrs((const void* const*)res, p->nfields, "res")) ||
+      (rc = check_ptrs((const void* const*)rhs, p->nfields, "rhs")) ||
+      (rc = check_ptrs((const void* const*)q32, p->nfields, "q32")))
+    return rc;
+  LserkMixedArgs s{};
+  for (int c = 0; c < p->nfields; ++c) {
+    s.q.p[c] = q[c];
+    s.res.p[c] = res[c];
+    s.rhs[c] = rhs[c];
+    s.q32[c] = q32[c];
+  }
+  s.n = p->nelem * p->np;
+  s.nf = p->nfields;
+  s.a = a;
+  s.b = b;
+  s.dt = dt;
+  hipLaunchKernelGGL(lserk_mixed_kernel, dim3(p->lserk_grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, s);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
 
 int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   g_last_error.clear();
@@ -287,6 +470,26 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   grid = (grid + 7) / 8 * 8;
   p->grid = (int)grid;
   p->lserk_grid = prop.multiProcessorCount * 8;
+  for (int k = 0; k < 2; ++k) {
+    const int prec = k == 0 ? DG_PREC_FP32 : DG_PREC_BF16X3;
+    std::vector<uint32_t> ow, lw;
+    build_mixed_tiles(prec, np, nfp, dsrc, d->lift, ow, lw);
+    if ((rc = upload(&p->mixed_op[k], ow.data(), ow.size())) ||
+        (rc = upload(&p->mixed_lift[k], lw.data(), lw.size()))) {
+      free_plan(p);
+      return rc;
+    }
+    p->mixed_kernel[k] = pick_mixed(prec, d->equation, d->order);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->mixed_kernel[k], kBlock,
+                                                      0) != hipSuccess ||
+        occ < 1)
+      occ = 1;
+    int64_t mg = (int64_t)prop.multiProcessorCount * occ;
+    const int64_t mbatch = (((d->nelements + 15) / 16) + 3) / 4;
+    if (mg > mbatch) mg = mbatch;
+    p->mixed_grid[k] = (int)((mg + 7) / 8 * 8);
+  }
   *out = p;
   return DG_OK;
 }

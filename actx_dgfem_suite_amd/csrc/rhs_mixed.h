@@ -1,0 +1,402 @@
+// Mixed-precision fused RHS kernel: FP32 state in and RHS out, the D and LIFT
+// contractions on the matrix cores in FP32 (v_mfma_f32_16x16x4f32) or as a
+// three-product bf16 split (v_mfma_f32_16x16x16bf16_1k) with FP32
+// accumulation, and everything else -- geometric factors, the numerical flux
+// (Euler pressure recovery included) and the combination of reference
+// derivatives into physical ones -- in FP64 on values promoted from FP32.
+//
+// The work decomposition is the FP64 kernel's (rhs_mfma.h): each wave owns
+// tiles of 16 elements (the MFMA N dimension), B operands come straight from
+// HBM, the flux is produced in a gather-friendly order and handed over
+// through a small per-wave LDS buffer, and outputs leave through the same
+// buffer as contiguous stores.
+//
+// Operand layouts (lane l, g = l / 16, n = l % 16), measured on gfx950:
+//   f32 16x16x4:       A[m = n][k = g], B[k = g][col = n], C[row = 4 g + v][col = n]
+//   bf16 16x16x16_1k:  lane l supplies 4 consecutive k = 4 g + i (i = 0..3) of
+//                      A[m = n][k] and B[k][col = n]; C as for f32.
+// Contraction index k at FP32 k-step ks stands for node j = g * KS + ks (a
+// contiguous run per lane group, as in the FP64 kernel); the bf16 form takes
+// four consecutive k-steps per instruction (KS rounded up to a multiple of 4).
+//
+// BF16x3: x = x_hi + x_lo with x_hi = bf16(x), x_lo = bf16(x - x_hi); the
+// product is a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, ~2^-16
+// relative, is dropped).  A operands are split on the host, B on the fly.
+#pragma once
+
+#include "dg_common.h"
+
+namespace dg {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+struct MixedArgs {
+  const float* q[kMaxFields];
+  float* rhs[kMaxFields];
+  const double* geo;        // (K, 25), FP64
+  const int32_t* emap;      // (K, 4*Nfp)
+  const int32_t* fmask;     // (4*Nfp)
+  const void* op_tiles;     // A tiles of D (float, or bf16 hi/lo pairs)
+  const void* lift_tiles;   // A tiles of LIFT
+  int64_t nelem;
+  double p0;
+};
+
+// round-to-nearest-even float -> bf16 bits (finite inputs)
+__host__ __device__ inline uint16_t bf16_bits(float x) {
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_value(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  __builtin_memcpy(&x, &u, 4);
+  return x;
+}
+
+template <int PREC>
+struct MixedMma;
+
+// FP32: one k per lane per instruction
+template <>
+struct MixedMma<DG_PREC_FP32> {
+  static constexpr int KG = 1;                  // k-steps per instruction
+  static constexpr int A_WORDS = 1;             // 32-bit words of A per lane
+  __device__ static int row(int g, int v) { return 4 * g + v; }
+  __device__ static f32x4 mma(const uint32_t* a, const float* b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[0]), b[0], c, 0, 0, 0);
+  }
+};
+
+// BF16x3: four k per lane per instruction, A as (hi[4], lo[4]) bf16
+template <>
+struct MixedMma<DG_PREC_BF16X3> {
+  static constexpr int KG = 4;
+  static constexpr int A_WORDS = 4;             // hi: 2 words, lo: 2 words
+  __device__ static int row(int g, int v) { return 4 * g + v; }
+  __device__ static f32x4 mma(const uint32_t* a, const float* b, f32x4 c) {
+    bf16x4 ahi, alo, bhi, blo;
+    ahi[0] = (short)(a[0] & 0xffff); ahi[1] = (short)(a[0] >> 16);
+    ahi[2] = (short)(a[1] & 0xffff); ahi[3] = (short)(a[1] >> 16);
+    alo[0] = (short)(a[2] & 0xffff); alo[1] = (short)(a[2] >> 16);
+    alo[2] = (short)(a[3] & 0xffff); alo[3] = (short)(a[3] >> 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint16_t hi = bf16_bits(b[i]);
+      bhi[i] = (short)hi;
+      blo[i] = (short)bf16_bits(b[i] - bf16_value(hi));
+    }
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, c, 0, 0, 0);
+  }
+};
+
+template <int PREC, int ORDER>
+struct MixedShape {
+  static constexpr int NP = np_of(ORDER);
+  static constexpr int NFP = nfp_of(ORDER);
+  static constexpr int NFN = kFaces * NFP;
+  static constexpr int MB = (NP + 15) / 16;
+  static constexpr int KG = MixedMma<PREC>::KG;
+  static constexpr int KS = ((NP + 3) / 4 + KG - 1) / KG * KG;  // node k-steps per lane group
+  static constexpr int KSL = (NFP + KG - 1) / KG * KG;         // LIFT k-steps per face
+  static constexpr int AW = MixedMma<PREC>::A_WORDS;
+  // A tiles, 32-bit words: [r][b][ks / KG][lane][AW] and [b][ks / KG][lane][AW]
+  static constexpr int OP_WORDS = 3 * MB * (KS / KG) * 64 * AW;
+  static constexpr int LIFT_WORDS = MB * (KSL / KG) * 64 * AW;
+};
+
+constexpr int kMixedFluxRow = 65;
+
+template <int EQ, int PREC, int ORDER>
+__global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
+  using S = MixedShape<PREC, ORDER>;
+  using M = MixedMma<PREC>;
+  using Flux = typename FluxOf<EQ>::type;
+  constexpr int NP = S::NP, NFP = S::NFP, NFN = S::NFN, MB = S::MB;
+  constexpr int KS = S::KS, KSL = S::KSL, KG = S::KG, AW = S::AW;
+  constexpr int NF = nfields_of(EQ);
+  constexpr int WAVES = kBlock / 64;
+  constexpr int FROW = kMixedFluxRow;
+  // flux buffer: KSL k-steps x NF fields x FROW; output: 16 x NP of one field
+  constexpr int WB = (KSL * NF * FROW > 16 * NP) ? KSL * NF * FROW : 16 * NP;
+
+  __shared__ uint32_t s_op[S::OP_WORDS];
+  __shared__ uint32_t s_lift[S::LIFT_WORDS];
+  __shared__ int s_fmask[NFN];
+  __shared__ float s_buf[WAVES][WB];
+
+  const int tid = threadIdx.x;
+  {
+    const uint32_t* op = (const uint32_t*)a.op_tiles;
+    const uint32_t* lf = (const uint32_t*)a.lift_tiles;
+    for (int k = tid; k < S::OP_WORDS; k += kBlock) s_op[k] = op[k];
+    for (int k = tid; k < S::LIFT_WORDS; k += kBlock) s_lift[k] = lf[k];
+    for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
+  }
+  __syncthreads();
+
+  const int lane = tid & 63;
+  const int g = lane >> 4;
+  const int n = lane & 15;
+  const int64_t ntiles = (a.nelem + 15) / 16;
+  const int64_t nwaves = (int64_t)gridDim.x * WAVES;
+  const int64_t wid = (int64_t)logical_block() * WAVES + (tid >> 6);
+  const int64_t per = (ntiles + nwaves - 1) / nwaves;
+  const int64_t t_begin = wid * per;
+  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  const double c0 = a.p0;
+  float* const buf = s_buf[tid >> 6];
+
+  auto a_op = [&](int r, int b, int kq) {
+    return &s_op[((((r * MB + b) * (KS / KG)) + kq) * 64 + lane) * AW];
+  };
+  auto a_lift = [&](int b, int kq) {
+    return &s_lift[(((b * (KSL / KG)) + kq) * 64 + lane) * AW];
+  };
+
+  for (int64_t tile = t_begin; tile < t_end; ++tile) {
+    const int64_t gel = tile * 16 + n;
+    const bool valid = gel < a.nelem;
+    const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
+    const double* geo = a.geo + el * kGeo;
+    double rx[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rx[k] = geo[k];
+
+    f32x4 acc[NF][MB];
+#pragma unroll
+    for (int c = 0; c < NF; ++c)
+#pragma unroll
+      for (int b = 0; b < MB; ++b) acc[c][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // {{{ volume terms (B operands built in FP64, rounded once to FP32)
+
+#pragma unroll 1
+    for (int kq = 0; kq < KS / KG; ++kq) {
+      if constexpr (EQ == DG_EQ_WAVE) {
+        float ub[KG], wb[3][KG];
+#pragma unroll
+        for (int i = 0; i < KG; ++i) {
+          const int j = g * KS + kq * KG + i;
+          double u = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+          if (j < NP) {
+            const int64_t idx = el * NP + j;
+            u = a.q[0][idx];
+            vx = a.q[1][idx];
+            vy = a.q[2][idx];
+            vz = a.q[3][idx];
+          }
+          ub[i] = (float)u;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            wb[r][i] = (float)(rx[r] * vx + rx[3 + r] * vy + rx[6 + r] * vz);
+        }
+        // t_r = D_r u lives in acc[1 + r] until the combination below
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const uint32_t* av = a_op(r, b, kq);
+            uint32_t aw[AW];
+#pragma unroll
+            for (int w = 0; w < AW; ++w) aw[w] = av[w];
+            acc[1 + r][b] = M::mma(aw, ub, acc[1 + r][b]);
+            acc[0][b] = M::mma(aw, wb[r], acc[0][b]);
+          }
+      } else if constexpr (EQ == DG_EQ_EULER) {
+        const double gamma = c0;
+        float wb[5][3][KG];
+#pragma unroll
+        for (int i = 0; i < KG; ++i) {
+          const int j = g * KS + kq * KG + i;
+          if (j < NP) {
+            const int64_t idx = el * NP + j;
+            const double rho = a.q[0][idx], e = a.q[1][idx];
+            const double m[3] = {a.q[2][idx], a.q[3][idx], a.q[4][idx]};
+            const double inv_rho = 1.0 / rho;
+            const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
+            const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
+            const double h = (e + p) * inv_rho;
+            double f[5][3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+              f[0][x] = m[x];
+              f[1][x] = m[x] * h;
+#pragma unroll
+              for (int y = 0; y < 3; ++y) f[2 + y][x] = m[y] * u[x] + (x == y ? p : 0.0);
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+#pragma unroll
+              for (int r = 0; r < 3; ++r)
+                wb[c][r][i] = (float)(rx[r] * f[c][0] + rx[3 + r] * f[c][1] + rx[6 + r] * f[c][2]);
+          } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) wb[c][0][i] = wb[c][1][i] = wb[c][2][i] = 0.f;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const uint32_t* av = a_op(r, b, kq);
+            uint32_t aw[AW];
+#pragma unroll
+            for (int w = 0; w < AW; ++w) aw[w] = av[w];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) acc[c][b] = M::mma(aw, wb[c][r], acc[c][b]);
+          }
+      }
+    }
+    if constexpr (EQ == DG_EQ_WAVE) {
+#pragma unroll
+      for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double t0 = acc[1][b][v], t1 = acc[2][b][v], t2 = acc[3][b][v];
+          acc[0][b][v] = (float)(-c0 * (double)acc[0][b][v]);
+#pragma unroll
+          for (int x = 0; x < 3; ++x)
+            acc[1 + x][b][v] = (float)(-c0 * (rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2));
+        }
+    }
+    if constexpr (EQ == DG_EQ_MAXWELL) {
+      // E_t = curl H, H_t = -curl E from t_{c,r} = D_r F_c, one field at a time
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        f32x4 tr[3][MB];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < MB; ++b) tr[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int kq = 0; kq < KS / KG; ++kq) {
+          float ub[KG];
+#pragma unroll
+          for (int i = 0; i < KG; ++i) {
+            const int j = g * KS + kq * KG + i;
+            ub[i] = j < NP ? a.q[c][el * NP + j] : 0.f;
+          }
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+              const uint32_t* av = a_op(r, b, kq);
+              uint32_t aw[AW];
+#pragma unroll
+              for (int w = 0; w < AW; ++w) aw[w] = av[w];
+              tr[r][b] = M::mma(aw, ub, tr[r][b]);
+            }
+        }
+        const int comp = c % 3;
+        const bool is_h = c >= 3;
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double t0 = tr[0][b][v], t1 = tr[1][b][v], t2 = tr[2][b][v];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+              const double d = rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2;
+#pragma unroll
+              for (int o = 0; o < 3; ++o) {
+                const int sgn = (o == x || x == comp || o == comp) ? 0
+                                : (((x - o + 3) % 3 == 1) ? 1 : -1);
+                if (sgn == 0) continue;
+                const int out = is_h ? o : 3 + o;
+                acc[out][b][v] += (float)((is_h ? sgn : -sgn) * d);
+              }
+            }
+          }
+      }
+    }
+
+    // }}}
+
+    // {{{ surface terms: flux in FP64, staged as FP32 B operands
+
+#pragma unroll
+    for (int it = 0; it < KSL; ++it) {
+      const int item = it * 64 + lane;
+      const int kk = item % KSL;
+      const int p = item / KSL;                   // p = face * 16 + element
+      const int fg = p >> 4, fe = p & 15;
+      float outf[NF];
+      if (kk < NFP) {
+        int64_t pe = tile * 16 + fe;
+        pe = pe < a.nelem ? pe : a.nelem - 1;
+        const double* pgeo = a.geo + pe * kGeo + 9 + 4 * fg;
+        const double nrm[3] = {pgeo[0], pgeo[1], pgeo[2]};
+        const double fscale = pgeo[3];
+        const int fn = fg * NFP + kk;
+        const int64_t vn = pe * NP + s_fmask[fn];
+        double qm[NF], qp[NF], out[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qm[c] = a.q[c][vn];
+        const int32_t ext = a.emap[pe * NFN + fn];
+        if (ext >= 0) {
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qp[c] = a.q[c][ext];
+        } else {
+          Flux::boundary(qm, nrm, qp);
+        }
+        Flux::flux(qm, qp, nrm, c0, out);
+#pragma unroll
+        for (int c = 0; c < NF; ++c) outf[c] = (float)(fscale * out[c]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) outf[c] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < NF; ++c) buf[(kk * NF + c) * FROW + p] = outf[c];
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll 1
+    for (int kq = 0; kq < KSL / KG; ++kq) {
+#pragma unroll
+      for (int b = 0; b < MB; ++b) {
+        const uint32_t* av = a_lift(b, kq);
+        uint32_t aw[AW];
+#pragma unroll
+        for (int w = 0; w < AW; ++w) aw[w] = av[w];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+          float fb[KG];
+#pragma unroll
+          for (int i = 0; i < KG; ++i) fb[i] = buf[((kq * KG + i) * NF + c) * FROW + lane];
+          acc[c][b] = M::mma(aw, fb, acc[c][b]);
+        }
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+
+    // }}}
+
+    // {{{ output through the buffer, one field at a time
+
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+#pragma unroll
+      for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * b + M::row(g, v);
+          if (i < NP) buf[n * NP + i] = acc[c][b][v];
+        }
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      float* dst = a.rhs[c] + tile * 16 * NP;
+      for (int idx = lane; idx < 16 * NP; idx += 64)
+        if (tile * 16 + idx / NP < a.nelem) dst[idx] = buf[idx];
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
+
+    // }}}
+  }
+}
+
+}  // namespace dg

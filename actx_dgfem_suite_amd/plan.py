@@ -84,7 +84,8 @@ class DGPlan:
     def shape(self) -> tuple[int, int]:
         return (self.discr.nelements, self.discr.np)
 
-    def _check_fields(self, fields: Sequence[torch.Tensor], what: str) -> None:
+    def _check_fields(self, fields: Sequence[torch.Tensor], what: str,
+                      dtype: torch.dtype = torch.float64) -> None:
         if len(fields) != self.nfields:
             raise ValueError(f"{what}: expected {self.nfields} fields, got {len(fields)}")
         for k, t in enumerate(fields):
@@ -92,21 +93,27 @@ class DGPlan:
                 raise TypeError(f"{what}[{k}]: expected torch.Tensor, got {type(t)}")
             if t.device != self.device:
                 raise ValueError(f"{what}[{k}] on {t.device}, plan on {self.device}")
-            if t.dtype != torch.float64:
-                raise TypeError(f"{what}[{k}]: expected float64, got {t.dtype}")
+            if t.dtype != dtype:
+                raise TypeError(f"{what}[{k}]: expected {dtype}, got {t.dtype}")
             if tuple(t.shape) != self.shape or not t.is_contiguous():
                 raise ValueError(f"{what}[{k}]: expected contiguous {self.shape}, "
                                  f"got {tuple(t.shape)}")
 
     # }}}
 
-    def empty_fields(self) -> list[torch.Tensor]:
-        return [torch.empty(self.shape, dtype=torch.float64, device=self.device)
+    def empty_fields(self, dtype: torch.dtype = torch.float64) -> list[torch.Tensor]:
+        return [torch.empty(self.shape, dtype=dtype, device=self.device)
                 for _ in range(self.nfields)]
 
     def rhs(self, q: Sequence[torch.Tensor], out: Sequence[torch.Tensor] | None = None,
             precision: str = "fp64") -> list[torch.Tensor]:
-        """Launch one RHS evaluation on the current stream; returns ``out``."""
+        """Launch one RHS evaluation on the current stream; returns ``out``.
+
+        ``precision="fp64"``: float64 fields (the drop-in form).
+        ``"fp32"`` / ``"bf16x3"``: the mixed form on float32 fields.
+        """
+        if precision in _lib.MIXED_PRECISIONS:
+            return self.rhs_mixed(q, out, precision)
         if precision not in _lib.PRECISION_IDS:
             raise NotImplementedError(f"precision '{precision}'")
         self._check_fields(q, "q")
@@ -120,6 +127,41 @@ class DGPlan:
         _lib.check(self.lib, self.lib.dg_rhs(self._handle, _lib.PRECISION_IDS[precision],
                                              pin, pout, ctypes.c_void_p(stream)), "dg_rhs")
         return list(out)
+
+    def rhs_mixed(self, q: Sequence[torch.Tensor], out: Sequence[torch.Tensor] | None = None,
+                  precision: str = "fp32") -> list[torch.Tensor]:
+        """Mixed-precision RHS on float32 fields (``"fp32"`` or ``"bf16x3"``)."""
+        if precision not in _lib.MIXED_PRECISIONS:
+            raise NotImplementedError(f"mixed precision '{precision}'")
+        self._check_fields(q, "q", torch.float32)
+        if out is None:
+            out = self.empty_fields(torch.float32)
+        else:
+            self._check_fields(out, "out", torch.float32)
+        pin = _lib.pointer_array([t.data_ptr() for t in q])
+        pout = _lib.pointer_array([t.data_ptr() for t in out])
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_rhs_mixed(self._handle, _lib.PRECISION_IDS[precision],
+                                                   pin, pout, ctypes.c_void_p(stream)),
+                   "dg_rhs_mixed")
+        return list(out)
+
+    def lserk_stage_mixed(self, a: float, b: float, dt: float, q: Sequence[torch.Tensor],
+                          res: Sequence[torch.Tensor], rhs: Sequence[torch.Tensor],
+                          q32: Sequence[torch.Tensor]) -> None:
+        """FP64 state/residual update from an FP32 RHS; refreshes the FP32 shadow."""
+        self._check_fields(q, "q")
+        self._check_fields(res, "res")
+        self._check_fields(rhs, "rhs", torch.float32)
+        self._check_fields(q32, "q32", torch.float32)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_lserk_stage_mixed(
+            self._handle, float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in q]),
+            _lib.pointer_array([t.data_ptr() for t in res]),
+            _lib.pointer_array([t.data_ptr() for t in rhs]),
+            _lib.pointer_array([t.data_ptr() for t in q32]),
+            ctypes.c_void_p(stream)), "dg_lserk_stage_mixed")
 
     def lserk_stage(self, a: float, b: float, dt: float, q: Sequence[torch.Tensor],
                     res: Sequence[torch.Tensor], rhs: Sequence[torch.Tensor]) -> None:

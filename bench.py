@@ -115,8 +115,24 @@ def main():
     torch.cuda.set_device(dev)
     d = w.discretization()
     plan = DGPlan(d, w.equation, w.param, device=dev, variant=args.variant)
+    mixed = args.precision != "fp64"
     q = [torch.from_numpy(np.ascontiguousarray(f)).to(dev) for f in initial_state(w, d)]
-    out = plan.empty_fields()
+    field_err = None
+    if mixed:
+        # per-field relative L2 of the mixed form against the FP64 form
+        ref = plan.rhs(q)
+        q = [t.float() for t in q]
+        got = plan.rhs(q, precision=args.precision)
+        torch.cuda.synchronize()
+        # a field whose FP64 RHS is negligible (below 1e-3 of the whole RHS,
+        # e.g. the transverse momentum of a uniform x-flow) is measured against
+        # the whole RHS instead of its own round-off
+        total = float(torch.sqrt(sum(torch.sum(r * r) for r in ref)))
+        field_err = [float(torch.linalg.norm(g.double() - r)
+                           / max(float(torch.linalg.norm(r)), 1e-3 * total))
+                     for g, r in zip(got, ref)]
+        del ref, got
+    out = plan.empty_fields(torch.float32 if mixed else torch.float64)
     nf = len(q)
     k, npn, nfp = d.nelements, d.np, d.nfp
     print(f"[bench] {w.name}: K={k} Np={npn} fields={nf} setup done", file=sys.stderr,
@@ -148,7 +164,7 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {n * 8} bytes read + written x3", file=sys.stderr)
 
-    b_alg = algorithmic_bytes(k, npn, nfp, nf)
+    b_alg = algorithmic_bytes(k, npn, nfp, nf, state_bytes=4 if mixed else 8)
     f_alg = algorithmic_flops(w.equation, k, npn, nfp, nf)
     t_kernel = kernel_ms * 1e-3
     achieved = b_alg / t_kernel / 1e9
@@ -166,7 +182,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if not mixed else ("f32" if args.precision == "fp32" else "bf16x3/f32"),
         "data": "synthetic",
         "config": {
             "workload": w.name,
@@ -188,13 +204,16 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"rhs_{'lds' if args.variant == 'lds' else 'mfma'}_kernel<{w.equation},P{w.order}>",
+            "kernel": (f"rhs_mixed_kernel<{w.equation},{args.precision},P{w.order}>" if mixed else
+                       f"rhs_{'lds' if args.variant == 'lds' else 'mfma'}_kernel<{w.equation},P{w.order}>"),
             "kernel_us": kernel_ms * 1e3,
             "algorithmic_bytes": b_alg,
             "algorithmic_flops": f_alg,
             "combined_frac": t_roof / t_kernel,
         },
     }
+    if field_err is not None:
+        result["mixed_rel_l2_vs_fp64"] = field_err
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline sample ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)

@@ -52,7 +52,9 @@ typedef enum dg_kernel_variant {
 } dg_kernel_variant;
 
 typedef enum dg_precision {
-  DG_PREC_FP64 = 0    /* the drop-in form */
+  DG_PREC_FP64 = 0,   /* the drop-in form: FP64 state, FP64 arithmetic */
+  DG_PREC_FP32 = 1,   /* mixed: FP32 state, D/LIFT on FP32 MFMA, geometry/flux in FP64 */
+  DG_PREC_BF16X3 = 2  /* mixed: as FP32 with D/LIFT as a 3-product bf16 split */
 } dg_precision;
 
 /* Everything a plan needs; all pointers are HOST memory and are copied. */
@@ -107,6 +109,17 @@ int dg_rhs(const dg_plan* plan, int32_t precision,
            const double* const* fields_in, double* const* fields_out,
            void* stream);
 
+/* Mixed-precision right-hand side (precision DG_PREC_FP32 or DG_PREC_BF16X3):
+ * the same operator as dg_rhs on FP32 state, FP32 RHS out.  The D and LIFT
+ * contractions run on the matrix cores in FP32 (or a bf16x3 split) with FP32
+ * accumulation; geometric factors, the numerical flux (Euler pressure
+ * included) and the combination of reference into physical derivatives run
+ * in FP64.  It has no counterpart in the reference, whose array contexts
+ * evaluate the operator in the state's own precision. */
+int dg_rhs_mixed(const dg_plan* plan, int32_t precision,
+                 const float* const* fields_in, float* const* fields_out,
+                 void* stream);
+
 /* One stage of the five-stage, fourth-order 2N-storage Runge-Kutta scheme
  * (Carpenter & Kennedy 1994), as in libparanumal's LSERK4 that the reference
  * compares against (assets/h200nvl.patch:42):
@@ -115,6 +128,14 @@ int dg_rhs(const dg_plan* plan, int32_t precision,
 int dg_lserk_stage(const dg_plan* plan, double a, double b, double dt,
                    double* const* q, double* const* res,
                    const double* const* rhs, void* stream);
+
+/* LSERK4 stage for the mixed form: FP64 q and res, FP32 rhs in, and the FP32
+ * shadow q32 = (float)q written for the next mixed RHS evaluation.  No
+ * counterpart in the reference (see dg_lserk_stage). */
+int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
+                         double* const* q, double* const* res,
+                         const float* const* rhs, float* const* q32,
+                         void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* dg_last_error_string(void);
