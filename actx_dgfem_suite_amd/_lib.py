@@ -31,6 +31,8 @@ EXPORTED_SYMBOLS = (
     "dg_rhs_mixed",
     "dg_lserk_stage",
     "dg_lserk_stage_mixed",
+    "dg_rhs_vjp",
+    "dg_lserk_stage_adjoint",
     "dg_last_error_string",
 )
 
@@ -49,6 +51,8 @@ class DgPlanDesc(ctypes.Structure):
         ("params", ctypes.c_double * 4),
         ("variant", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("mass_inv", ctypes.POINTER(ctypes.c_double)),
+        ("face_mass", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -82,6 +86,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_lserk_stage_mixed.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _PP, _PP, _PP, _PP, ctypes.c_void_p]
     lib.dg_lserk_stage_mixed.restype = ctypes.c_int
+    lib.dg_rhs_vjp.argtypes = [ctypes.c_void_p, _PP, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_rhs_vjp.restype = ctypes.c_int
+    lib.dg_lserk_stage_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_lserk_stage_adjoint.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []
     lib.dg_last_error_string.restype = ctypes.c_char_p
     return lib

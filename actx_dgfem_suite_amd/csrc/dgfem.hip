@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +28,7 @@
 #include "rhs_lds.h"
 #include "rhs_mfma.h"
 #include "rhs_mixed.h"
+#include "rhs_adjoint.h"
 
 namespace {
 
@@ -65,9 +67,38 @@ __global__ __launch_bounds__(kBlock) void lserk_stage_kernel(LserkArgs s) {
     double* res = s.res.p[f];
     const double* rhs = s.rhs.p[f];
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < s.n; k += stride) {
-      const double r = s.a * res[k] + s.dt * rhs[k];
+      // a == 0 (the first stage) does not read res, so it may start uninitialised
+      const double r = (s.a != 0.0 ? s.a * res[k] : 0.0) + s.dt * rhs[k];
       res[k] = r;
       q[k] += s.b * r;
+    }
+  }
+}
+
+// Reverse sweep of one stage: q_bar += dt*w (w optional); s = a*s + b*q_bar
+// (s not read when a == 0).
+struct LserkAdjArgs {
+  OutPtrs qbar;
+  OutPtrs s;
+  InPtrs w;
+  int64_t n;
+  int nf;
+  double a, b, dt;
+};
+
+__global__ __launch_bounds__(kBlock) void lserk_adjoint_kernel(LserkAdjArgs s) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int f = 0; f < s.nf; ++f) {
+    double* qb = s.qbar.p[f];
+    double* sv = s.s.p[f];
+    const double* w = s.w.p[f];
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < s.n; k += stride) {
+      double q = qb[k];
+      if (w) {
+        q += s.dt * w[k];
+        qb[k] = q;
+      }
+      sv[k] = (s.a != 0.0 ? s.a * sv[k] : 0.0) + s.b * q;
     }
   }
 }
@@ -145,6 +176,105 @@ MixedKernel pick_mixed(int prec, int eq, int order) {
   return nullptr;
 }
 
+// {{{ adjoint kernels and their host-side data
+
+using VjpKernel = void (*)(VjpArgs);
+using MassInvKernel = void (*)(InPtrs, OutPtrs, int, const double*, int64_t);
+
+template <int EQ>
+VjpKernel pick_vjp_order(int order) {
+  switch (order) {
+    case 1: return rhs_vjp_kernel<EQ, 1>;
+    case 2: return rhs_vjp_kernel<EQ, 2>;
+    case 3: return rhs_vjp_kernel<EQ, 3>;
+    case 4: return rhs_vjp_kernel<EQ, 4>;
+    default: return nullptr;
+  }
+}
+
+VjpKernel pick_vjp(int eq, int order) {
+  switch (eq) {
+    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE>(order);
+    case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER>(order);
+    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL>(order);
+    default: return nullptr;
+  }
+}
+
+MassInvKernel pick_mass_inv(int order) {
+  switch (order) {
+    case 1: return mass_inv_kernel<1>;
+    case 2: return mass_inv_kernel<2>;
+    case 3: return mass_inv_kernel<3>;
+    case 4: return mass_inv_kernel<4>;
+    default: return nullptr;
+  }
+}
+
+// Host data of the adjoint: per-face blocks of the face mass, the face nodes
+// of every volume node, and the neighbour's Fscale across every face.  Fails
+// (returns a message) when E couples nodes off its face or the exterior map
+// does not pair whole faces.
+std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp,
+                               std::vector<double>& face_mass, std::vector<int32_t>& node_faces,
+                               std::vector<double>& ext_fscale) {
+  const int nfn = kFaces * nfp;
+  const int64_t k = d->nelements;
+  const int* fmask = d->face_nodes;
+  double emax = 0.0;
+  for (int t = 0; t < np * nfn; ++t) emax = std::fmax(emax, std::fabs(d->face_mass[t]));
+  std::vector<int> on_face((size_t)kFaces * np, -1);  // face-node index b of node i on face f
+  for (int f = 0; f < kFaces; ++f)
+    for (int b = 0; b < nfp; ++b) on_face[f * np + fmask[f * nfp + b]] = b;
+  face_mass.assign((size_t)kFaces * nfp * nfp, 0.0);
+  for (int f = 0; f < kFaces; ++f)
+    for (int i = 0; i < np; ++i)
+      for (int a = 0; a < nfp; ++a) {
+        const double v = d->face_mass[(size_t)i * nfn + f * nfp + a];
+        const int b = on_face[f * np + i];
+        if (b < 0) {
+          if (std::fabs(v) > 1e-12 * emax) return "face_mass couples a node off its face";
+        } else {
+          face_mass[((size_t)f * nfp + b) * nfp + a] = v;
+        }
+      }
+  node_faces.assign((size_t)np * 3, -1);
+  std::vector<int> bits(np, 0);
+  for (int f = 0; f < kFaces; ++f)
+    for (int b = 0; b < nfp; ++b) {
+      const int i = fmask[f * nfp + b];
+      int s = 0;
+      while (s < 3 && node_faces[i * 3 + s] >= 0) ++s;
+      if (s == 3) return "face_nodes: a node lies on more than three faces";
+      node_faces[i * 3 + s] = f * nfp + b;
+      bits[i] |= 1 << f;
+    }
+  ext_fscale.assign((size_t)k * kFaces, 0.0);
+  for (int64_t e = 0; e < k; ++e)
+    for (int f = 0; f < kFaces; ++f) {
+      const int32_t* em = d->ext_map + e * nfn + f * nfp;
+      if (em[0] < 0) {
+        for (int a = 1; a < nfp; ++a)
+          if (em[a] >= 0) return "ext_map mixes boundary and interior nodes on one face";
+        continue;
+      }
+      const int64_t e2 = em[0] / np;
+      int mask = 0xF;
+      for (int a = 0; a < nfp; ++a) {
+        if (em[a] < 0 || em[a] / np != e2) return "ext_map does not pair whole faces";
+        mask &= bits[em[a] % np];
+      }
+      if (mask == 0 || (mask & (mask - 1)) != 0)
+        return "ext_map: exterior nodes of a face do not lie on one face";
+      int f2 = 0;
+      while (!(mask & (1 << f2))) ++f2;
+      ext_fscale[e * kFaces + f] = d->geometry[e2 * kGeo + 9 + 4 * f2 + 3];
+    }
+  return "";
+}
+
+// }}}
+
 // A tiles of the mixed kernel (layout in rhs_mixed.h): 32-bit words, either
 // one float per lane (FP32) or four (hi, lo) bf16 pairs packed in 4 words.
 void build_mixed_tiles(int prec, int np, int nfp, const double* op, const double* lift,
@@ -221,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
     const float* rhs = s.rhs[f];
     float* q32 = s.q32[f];
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < s.n; k += stride) {
-      const double r = s.a * res[k] + s.dt * (double)rhs[k];
+      const double r = (s.a != 0.0 ? s.a * res[k] : 0.0) + s.dt * (double)rhs[k];
       res[k] = r;
       const double qn = q[k] + s.b * r;
       q[k] = qn;
@@ -281,6 +411,17 @@ struct dg_plan {
   uint32_t* mixed_lift[2] = {nullptr, nullptr};
   MixedKernel mixed_kernel[2] = {nullptr, nullptr};
   int mixed_grid[2] = {0, 0};
+  // adjoint (present when the descriptor carried mass_inv and face_mass)
+  bool has_vjp = false;
+  VjpKernel vjp_kernel = nullptr;
+  MassInvKernel massinv_kernel = nullptr;
+  int vjp_grid = 0;
+  int massinv_grid = 0;
+  double* dmat = nullptr;        // (3, Np, Np) untransposed D
+  double* face_mass = nullptr;   // (4, Nfp, Nfp)
+  double* mass_inv = nullptr;    // (Np, Np)
+  double* ext_fscale = nullptr;  // (K, 4)
+  int32_t* node_faces = nullptr; // (Np, 3)
 };
 
 namespace {
@@ -298,6 +439,11 @@ void free_plan(dg_plan* p) {
     (void)hipFree(p->mixed_op[k]);
     (void)hipFree(p->mixed_lift[k]);
   }
+  (void)hipFree(p->dmat);
+  (void)hipFree(p->face_mass);
+  (void)hipFree(p->mass_inv);
+  (void)hipFree(p->ext_fscale);
+  (void)hipFree(p->node_faces);
   delete p;
 }
 
@@ -413,6 +559,16 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   for (int t = 0; t < nfn; ++t)
     if (d->face_nodes[t] < 0 || d->face_nodes[t] >= np)
       return fail(DG_ERR_INVALID_ARGUMENT, "face_nodes entry out of range");
+  // adjoint data, built and checked before anything touches the device
+  const bool want_vjp = d->mass_inv || d->face_mass;
+  std::vector<double> fm, xfs;
+  std::vector<int32_t> nf;
+  if (want_vjp) {
+    if (!d->mass_inv || !d->face_mass)
+      return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv and face_mass must be given together");
+    const std::string msg = build_adjoint_data(d, np, nfp, fm, nf, xfs);
+    if (!msg.empty()) return fail(DG_ERR_INVALID_ARGUMENT, msg);
+  }
 
   dg_plan* p = new (std::nothrow) dg_plan;
   if (!p) return fail(DG_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -490,6 +646,31 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     if (mg > mbatch) mg = mbatch;
     p->mixed_grid[k] = (int)((mg + 7) / 8 * 8);
   }
+  if (want_vjp) {
+    if ((rc = upload(&p->dmat, dsrc, (size_t)3 * np * np)) ||
+        (rc = upload(&p->face_mass, fm.data(), fm.size())) ||
+        (rc = upload(&p->mass_inv, d->mass_inv, (size_t)np * np)) ||
+        (rc = upload(&p->ext_fscale, xfs.data(), xfs.size())) ||
+        (rc = upload(&p->node_faces, nf.data(), nf.size()))) {
+      free_plan(p);
+      return rc;
+    }
+    p->vjp_kernel = pick_vjp(d->equation, d->order);
+    p->massinv_kernel = pick_mass_inv(d->order);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->vjp_kernel, kBlock, 0) !=
+            hipSuccess ||
+        occ < 1)
+      occ = 1;
+    const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
+    int64_t vg = (int64_t)prop.multiProcessorCount * occ;
+    if (vg > vbatch) vg = vbatch;
+    p->vjp_grid = (int)((vg + 7) / 8 * 8);
+    int64_t mg = (int64_t)prop.multiProcessorCount * 4;
+    if (mg > vbatch) mg = vbatch;
+    p->massinv_grid = (int)mg;
+    p->has_vjp = true;
+  }
   *out = p;
   return DG_OK;
 }
@@ -557,6 +738,70 @@ int dg_lserk_stage(const dg_plan* p, double a, double b, double dt, double* cons
   s.b = b;
   s.dt = dt;
   hipLaunchKernelGGL(lserk_stage_kernel, dim3(p->lserk_grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, s);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
+int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rhs_bar,
+               double* const* q_bar, double* const* workspace, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->has_vjp)
+    return fail(DG_ERR_UNSUPPORTED, "plan created without mass_inv / face_mass: no adjoint");
+  int rc;
+  if (p->equation == DG_EQ_EULER && (rc = check_ptrs((const void* const*)q, p->nfields, "q")))
+    return rc;
+  if ((rc = check_ptrs((const void* const*)rhs_bar, p->nfields, "rhs_bar")) ||
+      (rc = check_ptrs((const void* const*)q_bar, p->nfields, "q_bar")) ||
+      (rc = check_ptrs((const void* const*)workspace, p->nfields, "workspace")))
+    return rc;
+  VjpArgs a{};
+  OutPtrs mu{};
+  for (int c = 0; c < p->nfields; ++c) {
+    a.q.p[c] = q ? q[c] : nullptr;
+    a.lam.p[c] = rhs_bar[c];
+    a.mu.p[c] = workspace[c];
+    a.qbar.p[c] = q_bar[c];
+    mu.p[c] = workspace[c];
+  }
+  hipLaunchKernelGGL(p->massinv_kernel, dim3(p->massinv_grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, a.lam, mu, p->nfields, p->mass_inv, p->nelem);
+  DG_HIP_CHECK(hipGetLastError());
+  a.geo = p->geo;
+  a.emap = p->emap;
+  a.fmask = p->fmask;
+  a.dmat = p->dmat;
+  a.face_mass = p->face_mass;
+  a.mass_inv = p->mass_inv;
+  a.ext_fscale = p->ext_fscale;
+  a.node_faces = p->node_faces;
+  a.nelem = p->nelem;
+  a.p0 = p->params[0];
+  hipLaunchKernelGGL(p->vjp_kernel, dim3(p->vjp_grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
+int dg_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt, double* const* qbar,
+                           double* const* sv, const double* const* w, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)qbar, p->nfields, "q_bar")) ||
+      (rc = check_ptrs((const void* const*)sv, p->nfields, "s")) ||
+      (w && (rc = check_ptrs((const void* const*)w, p->nfields, "w"))))
+    return rc;
+  LserkAdjArgs s{};
+  for (int c = 0; c < p->nfields; ++c) {
+    s.qbar.p[c] = qbar[c];
+    s.s.p[c] = sv[c];
+    s.w.p[c] = w ? w[c] : nullptr;
+  }
+  s.n = p->nelem * p->np;
+  s.nf = p->nfields;
+  s.a = a;
+  s.b = b;
+  s.dt = dt;
+  hipLaunchKernelGGL(lserk_adjoint_kernel, dim3(p->lserk_grid), dim3(kBlock), 0,
                      (hipStream_t)stream, s);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;

@@ -1,0 +1,439 @@
+// Adjoint (vector-Jacobian product) of the DG right-hand side:
+//   q_bar = (dL/dq)^T lam   for rhs = L(q)
+// Exact transpose of the forward kernels' arithmetic for wave and Maxwell
+// (linear), and of the Euler operator linearised at q (flux Jacobians
+// recomputed per node).
+//
+// Surface transpose without atomics.  The forward surface term is
+//   rhs_c[e,i] += sum_fn LIFT[i][fn] fs_e(f) Phi_c(qM, qP, n)
+// with LIFT = M^{-1} E and E the face mass.  Its transpose needs, per face
+// node, psi_e(fn) = fs_e(f) (E^T M^{-1} lam_e)(fn); E couples only the nodes of
+// one face, so with mu = M^{-1} lam precomputed per element (first kernel),
+//   psi_e(f, a) = fs_e(f) sum_b Mf_f[b][a] mu_e[fmask_f[b]].
+// The exterior map is an involution: the contribution of a neighbour's flux
+// to this element's face node is a GATHER of the neighbour's psi at the
+// matching face node, which in this element's face numbering is
+//   psi_e'(f, a) = fs_e'(f') sum_b Mf_f[b][a] mu_e'(ext(e, f, b))
+// (the face mass of one physical face is the same in both numberings), and
+//   q_bar(fmask[fn]) += JM^T psi_e + [boundary: B^T JP^T psi_e]
+//                     + JP(q_e'(ext), q_e(fn), -n)^T psi_e'.
+// Each element's face nodes are then summed into its volume nodes
+// (vertex nodes lie on three faces, edge nodes on two).
+//
+// Euler's LF wavespeed max(lam_M, lam_P) takes the subgradient that splits
+// exact ties 0.5 / 0.5 (the convention of JAX's maximum); ties occur at wall
+// faces, where the reflected state has the same |u| and c.
+#pragma once
+
+#include "dg_common.h"
+
+namespace dg {
+
+struct VjpArgs {
+  InPtrs q;          // state (Euler only; may be NULL for wave / Maxwell)
+  InPtrs lam;        // cotangent of the RHS
+  InPtrs mu;         // M^{-1} lam per element (workspace)
+  OutPtrs qbar;      // result
+  const double* geo;       // (K, 25)
+  const int32_t* emap;     // (K, 4*Nfp)
+  const int32_t* fmask;    // (4*Nfp)
+  const double* dmat;      // (3, Np, Np) D[r][j][i] row-major (weak, or strong for Maxwell)
+  const double* face_mass; // (4, Nfp, Nfp) Mf[f][b][a]
+  const double* mass_inv;  // (Np, Np) reference inverse mass
+  const double* ext_fscale;// (K, 4) Fscale of the neighbour across each face
+  const int32_t* node_faces;  // (Np, 3) face-node indices of each volume node, -1 padded
+  int64_t nelem;
+  double p0;
+};
+
+// {{{ flux VJPs: given psi (cotangent of the flux 'out' of FluxOf<EQ>::flux),
+// gM = (d out / d qM)^T psi, gP = (d out / d qP)^T psi; bcT: g <- B^T g for the
+// boundary state qP = B qM.
+
+struct WaveAdj {
+  static constexpr int NF = 4;
+  __device__ static void vjp(const double*, const double*, const double* n, double c,
+                             const double* psi, double* gM, double* gP) {
+    const double pvn = psi[1] * n[0] + psi[2] * n[1] + psi[3] * n[2];
+    gM[0] = 0.5 * c * (-psi[0] + pvn);
+    gP[0] = 0.5 * c * (psi[0] + pvn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gM[1 + k] = 0.5 * c * (psi[0] - pvn) * n[k];
+      gP[1 + k] = 0.5 * c * (psi[0] + pvn) * n[k];
+    }
+  }
+  __device__ static void bcT(const double*, double* g) { g[0] = -g[0]; }
+};
+
+struct MaxwellAdj {
+  static constexpr int NF = 6;
+  __device__ static void cross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  }
+  __device__ static void vjp(const double*, const double*, const double* n, double alpha,
+                             const double* psi, double* gM, double* gP) {
+    double ne[3], nh[3], nne[3], nnh[3];
+    cross(n, psi, ne);        // n x psi_E
+    cross(n, psi + 3, nh);    // n x psi_H
+    cross(n, ne, nne);
+    cross(n, nh, nnh);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double de = -0.5 * alpha * nne[k] + 0.5 * nh[k];
+      const double dh = -0.5 * ne[k] - 0.5 * alpha * nnh[k];
+      gP[k] = de;
+      gP[3 + k] = dh;
+      gM[k] = -de;
+      gM[3 + k] = -dh;
+    }
+  }
+  __device__ static void bcT(const double*, double* g) {
+    g[0] = -g[0];
+    g[1] = -g[1];
+    g[2] = -g[2];
+  }
+};
+
+struct EulerAdj {
+  static constexpr int NF = 5;
+  // g += (d (F(q).n) / d q)^T psi, n need not be a unit vector
+  __device__ static void nflux_vjp(const double* q, const double* n, double gamma,
+                                   const double* psi, double* g) {
+    const double rho = q[0], e = q[1];
+    const double m[3] = {q[2], q[3], q[4]};
+    const double inv = 1.0 / rho;
+    const double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    const double p = (gamma - 1.0) * (e - 0.5 * m2 * inv);
+    const double un = (m[0] * n[0] + m[1] * n[1] + m[2] * n[2]) * inv;
+    const double pmn = psi[2] * n[0] + psi[3] * n[1] + psi[4] * n[2];
+    const double pmm = psi[2] * m[0] + psi[3] * m[1] + psi[4] * m[2];
+    const double Q = psi[1] * (e + p) + pmm;
+    const double R = psi[1] * un + pmn;
+    g[0] += R * (gamma - 1.0) * 0.5 * m2 * inv * inv - Q * un * inv;
+    g[1] += psi[1] * un + (gamma - 1.0) * R;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      g[2 + k] += psi[0] * n[k] + Q * inv * n[k] + un * psi[2 + k] - (gamma - 1.0) * R * m[k] * inv;
+  }
+  // wavespeed |u| + c and its gradient
+  __device__ static double speed(const double* q, double gamma, double* ds) {
+    const double rho = q[0], e = q[1];
+    const double inv = 1.0 / rho;
+    const double u[3] = {q[2] * inv, q[3] * inv, q[4] * inv};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double p = (gamma - 1.0) * (e - 0.5 * rho * uu);
+    const double c = sqrt(gamma * p * inv);
+    const double au = sqrt(uu);
+    const double dp_drho = (gamma - 1.0) * 0.5 * uu;
+    const double k = gamma / (2.0 * c * rho);
+    ds[0] = (au > 0.0 ? -au * inv : 0.0) + k * (dp_drho - p * inv);
+    ds[1] = k * (gamma - 1.0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      ds[2 + j] = (au > 0.0 ? u[j] / (au * rho) : 0.0) - k * (gamma - 1.0) * u[j];
+    return au + c;
+  }
+  __device__ static void vjp(const double* qM, const double* qP, const double* n, double gamma,
+                             const double* psi, double* gM, double* gP) {
+    double dsM[5], dsP[5];
+    const double lM = speed(qM, gamma, dsM);
+    const double lP = speed(qP, gamma, dsP);
+    const double lam = fmax(lM, lP);
+    const double wM = lM > lP ? 1.0 : (lM < lP ? 0.0 : 0.5);
+    const double wP = 1.0 - wM;
+    double jump = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) jump += psi[c] * (qP[c] - qM[c]);
+    double hpsi[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      hpsi[c] = -0.5 * psi[c];
+      gM[c] = -0.5 * lam * psi[c] + 0.5 * jump * wM * dsM[c];
+      gP[c] = 0.5 * lam * psi[c] + 0.5 * jump * wP * dsP[c];
+    }
+    nflux_vjp(qM, n, gamma, hpsi, gM);
+    nflux_vjp(qP, n, gamma, hpsi, gP);
+  }
+  __device__ static void bcT(const double* n, double* g) {
+    const double gn = g[2] * n[0] + g[3] * n[1] + g[4] * n[2];
+    g[2] -= 2.0 * gn * n[0];
+    g[3] -= 2.0 * gn * n[1];
+    g[4] -= 2.0 * gn * n[2];
+  }
+};
+
+template <int EQ> struct AdjOf;
+template <> struct AdjOf<DG_EQ_WAVE> { using type = WaveAdj; };
+template <> struct AdjOf<DG_EQ_EULER> { using type = EulerAdj; };
+template <> struct AdjOf<DG_EQ_MAXWELL> { using type = MaxwellAdj; };
+
+// }}}
+
+// {{{ mu = M^{-1} lam per element (reference inverse mass)
+
+template <int ORDER>
+__global__ __launch_bounds__(kBlock) void mass_inv_kernel(InPtrs lam, OutPtrs mu, int nf,
+                                                          const double* mass_inv, int64_t nelem) {
+  constexpr int NP = np_of(ORDER);
+  constexpr int EB = kBlock / NP;
+  __shared__ double s_m[NP * NP];
+  __shared__ double s_l[EB * NP];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < NP * NP; k += kBlock) s_m[k] = mass_inv[k];
+  const int el = tid / NP, i = tid - el * NP;
+  const int64_t nbatch = (nelem + EB - 1) / EB;
+  for (int c = 0; c < nf; ++c) {
+    for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+      const int64_t e0 = b * EB;
+      const int nel = (int)(nelem - e0 < EB ? nelem - e0 : EB);
+      __syncthreads();
+      for (int k = tid; k < nel * NP; k += kBlock) s_l[k] = lam.p[c][e0 * NP + k];
+      __syncthreads();
+      if (el < nel) {
+        double acc = 0.0;
+#pragma unroll 5
+        for (int j = 0; j < NP; ++j) acc += s_m[i * NP + j] * s_l[el * NP + j];
+        mu.p[c][e0 * NP + tid] = acc;
+      }
+    }
+  }
+}
+
+// }}}
+
+// {{{ adjoint kernel: one thread per (element, node), element batches in LDS
+
+template <int EQ, int ORDER>
+__global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
+  using Adj = typename AdjOf<EQ>::type;
+  using Flux = typename FluxOf<EQ>::type;
+  constexpr int NP = np_of(ORDER);
+  constexpr int NFP = nfp_of(ORDER);
+  constexpr int NF = nfields_of(EQ);
+  constexpr int NFN = kFaces * NFP;
+  constexpr int EB = kBlock / NP;
+  constexpr bool NEEDS_Q = EQ == DG_EQ_EULER;
+  constexpr int NQ = NEEDS_Q ? NF : 1;
+  constexpr int NW = EQ == DG_EQ_WAVE ? 3 : 1;
+
+  __shared__ double s_d[3 * NP * NP];        // D[r][j][i]
+  __shared__ double s_mf[kFaces * NFP * NFP];
+  __shared__ int s_fmask[NFN];
+  __shared__ int s_nf[NP * 3];
+  __shared__ double s_lam[NF][EB * NP];
+  __shared__ double s_mu[NF][EB * NP];
+  __shared__ double s_q[NQ][EB * NP];
+  __shared__ double s_w[NW][EB * NP];
+  __shared__ double s_mux[NF][EB * NFN];     // mu at exterior nodes
+  __shared__ double s_qx[NQ][EB * NFN];      // q at exterior nodes (Euler)
+  __shared__ double s_g[NF][EB * NFN];       // face-node contributions to q_bar
+  __shared__ double s_geo[EB * kGeo];
+  __shared__ double s_xfs[EB * kFaces];
+
+  const int tid = threadIdx.x;
+  for (int k = tid; k < 3 * NP * NP; k += kBlock) s_d[k] = a.dmat[k];
+  for (int k = tid; k < kFaces * NFP * NFP; k += kBlock) s_mf[k] = a.face_mass[k];
+  for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
+  for (int k = tid; k < NP * 3; k += kBlock) s_nf[k] = a.node_faces[k];
+
+  const int64_t nbatch = (a.nelem + EB - 1) / EB;
+  const int64_t per = (nbatch + gridDim.x - 1) / gridDim.x;
+  const int64_t b_begin = (int64_t)logical_block() * per;
+  const int64_t b_end = b_begin + per < nbatch ? b_begin + per : nbatch;
+  const int el = tid / NP;
+  const int i = tid - el * NP;
+  const double c0 = a.p0;
+
+  for (int64_t b = b_begin; b < b_end; ++b) {
+    const int64_t e0 = b * EB;
+    const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NF; ++c)
+      for (int k = tid; k < nel * NP; k += kBlock) {
+        s_lam[c][k] = a.lam.p[c][e0 * NP + k];
+        s_mu[c][k] = a.mu.p[c][e0 * NP + k];
+        if constexpr (NEEDS_Q) s_q[c][k] = a.q.p[c][e0 * NP + k];
+      }
+    for (int k = tid; k < nel * kGeo; k += kBlock) s_geo[k] = a.geo[e0 * kGeo + k];
+    for (int k = tid; k < nel * kFaces; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * kFaces + k];
+    // exterior gathers
+    for (int k = tid; k < nel * NFN; k += kBlock) {
+      const int32_t ext = a.emap[e0 * NFN + k];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        s_mux[c][k] = ext >= 0 ? a.mu.p[c][ext] : 0.0;
+        if constexpr (NEEDS_Q) s_qx[c][k] = ext >= 0 ? a.q.p[c][ext] : 0.0;
+      }
+    }
+    __syncthreads();
+
+    const bool active = el < nel;
+    if constexpr (EQ == DG_EQ_WAVE) {
+      if (active) {
+        const double* g = &s_geo[el * kGeo];
+        const int node = el * NP + i;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          s_w[r][node] = g[r] * s_lam[1][node] + g[3 + r] * s_lam[2][node] + g[6 + r] * s_lam[3][node];
+      }
+    }
+
+    // face-node contributions
+    for (int k = tid; k < nel * NFN; k += kBlock) {
+      const int fe = k / NFN;
+      const int fn = k - fe * NFN;
+      const int f = fn / NFP;
+      const int aa = fn - f * NFP;
+      const double* geo = &s_geo[fe * kGeo + 9 + 4 * f];
+      const double n[3] = {geo[0], geo[1], geo[2]};
+      const double nm[3] = {-n[0], -n[1], -n[2]};
+      const double fs = geo[3];
+      const double xfs = s_xfs[fe * kFaces + f];
+      double psi[NF], psin[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        double so = 0.0, sn = 0.0;
+        for (int bb = 0; bb < NFP; ++bb) {
+          const double m = s_mf[(f * NFP + bb) * NFP + aa];
+          so += m * s_mu[c][fe * NP + s_fmask[f * NFP + bb]];
+          sn += m * s_mux[c][fe * NFN + f * NFP + bb];
+        }
+        psi[c] = fs * so;
+        psin[c] = xfs * sn;
+      }
+      const int32_t ext = a.emap[(e0 + fe) * NFN + fn];
+      double qm[NF], qp[NF], gM[NF], gP[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        qm[c] = NEEDS_Q ? s_q[c < NQ ? c : 0][fe * NP + s_fmask[fn]] : 0.0;
+        qp[c] = NEEDS_Q ? s_qx[c < NQ ? c : 0][k] : 0.0;
+      }
+      if (ext < 0) {
+        if constexpr (NEEDS_Q) Flux::boundary(qm, n, qp);
+        Adj::vjp(qm, qp, n, c0, psi, gM, gP);
+        Adj::bcT(n, gP);
+#pragma unroll
+        for (int c = 0; c < NF; ++c) gM[c] += gP[c];
+      } else {
+        Adj::vjp(qm, qp, n, c0, psi, gM, gP);
+        double hM[NF], hP[NF];
+        // the neighbour's flux: its interior is qp, its exterior is our qm
+        Adj::vjp(qp, qm, nm, c0, psin, hM, hP);
+#pragma unroll
+        for (int c = 0; c < NF; ++c) gM[c] += hP[c];
+      }
+#pragma unroll
+      for (int c = 0; c < NF; ++c) s_g[c][k] = gM[c];
+    }
+    __syncthreads();
+
+    if (active) {
+      const double* g = &s_geo[el * kGeo];
+      const int base = el * NP;
+      double acc[NF];
+      if constexpr (EQ == DG_EQ_WAVE) {
+        // u_bar = -c sum_r D_r^T (sum_x rx[x][r] lam_vx); v_x_bar = -c sum_r rx[x][r] D_r^T lam_u
+        double au = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int j = 0; j < NP; ++j) {
+          const double d0 = s_d[j * NP + i];
+          const double d1 = s_d[NP * NP + j * NP + i];
+          const double d2 = s_d[2 * NP * NP + j * NP + i];
+          const double lu = s_lam[0][base + j];
+          au += d0 * s_w[0][base + j] + d1 * s_w[1][base + j] + d2 * s_w[2][base + j];
+          t0 += d0 * lu;
+          t1 += d1 * lu;
+          t2 += d2 * lu;
+        }
+        acc[0] = -c0 * au;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[1 + x] = -c0 * (g[3 * x] * t0 + g[3 * x + 1] * t1 + g[3 * x + 2] * t2);
+      } else if constexpr (EQ == DG_EQ_EULER) {
+        // T_{c,r} = D_r^T lam_c; Fbar_{c,x} = sum_r rx[x][r] T_{c,r}; q_bar = sum_x (dF_x/dq)^T Fbar_x
+        double t[5][3];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) t[c][0] = t[c][1] = t[c][2] = 0.0;
+        for (int j = 0; j < NP; ++j) {
+          const double d0 = s_d[j * NP + i];
+          const double d1 = s_d[NP * NP + j * NP + i];
+          const double d2 = s_d[2 * NP * NP + j * NP + i];
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            const double l = s_lam[c][base + j];
+            t[c][0] += d0 * l;
+            t[c][1] += d1 * l;
+            t[c][2] += d2 * l;
+          }
+        }
+        double q[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          q[c] = s_q[c][base + i];
+          acc[c] = 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          double fb[5];
+#pragma unroll
+          for (int c = 0; c < 5; ++c) fb[c] = g[3 * x] * t[c][0] + g[3 * x + 1] * t[c][1] + g[3 * x + 2] * t[c][2];
+          const double ex[3] = {x == 0 ? 1.0 : 0.0, x == 1 ? 1.0 : 0.0, x == 2 ? 1.0 : 0.0};
+          EulerAdj::nflux_vjp(q, ex, c0, fb, acc);
+        }
+      } else {
+        // H_bar_a = sum eps_{oxa} d_x^T lam_{E_o}; E_bar_a = -sum eps_{oxa} d_x^T lam_{H_o}
+        double t[6][3];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t[c][0] = t[c][1] = t[c][2] = 0.0;
+        for (int j = 0; j < NP; ++j) {
+          const double d0 = s_d[j * NP + i];
+          const double d1 = s_d[NP * NP + j * NP + i];
+          const double d2 = s_d[2 * NP * NP + j * NP + i];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            const double l = s_lam[c][base + j];
+            t[c][0] += d0 * l;
+            t[c][1] += d1 * l;
+            t[c][2] += d2 * l;
+          }
+        }
+        double d[6][3];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+          for (int x = 0; x < 3; ++x) d[c][x] = g[3 * x] * t[c][0] + g[3 * x + 1] * t[c][1] + g[3 * x + 2] * t[c][2];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = 0.0;
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+          for (int o = 0; o < 3; ++o)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+              const int s = (o == x || x == comp || o == comp) ? 0 : (((x - o + 3) % 3 == 1) ? 1 : -1);
+              if (s == 0) continue;
+              acc[3 + comp] += s * d[o][x];       // H_bar from lam_E
+              acc[comp] -= s * d[3 + o][x];       // E_bar from lam_H
+            }
+      }
+      // face-node contributions of this node
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int fn = s_nf[i * 3 + s];
+        if (fn >= 0) {
+#pragma unroll
+          for (int c = 0; c < NF; ++c) acc[c] += s_g[c][el * NFN + fn];
+        }
+      }
+      const int64_t out = e0 * NP + tid;
+#pragma unroll
+      for (int c = 0; c < NF; ++c) a.qbar.p[c][out] = acc[c];
+    }
+  }
+}
+
+// }}}
+
+}  // namespace dg

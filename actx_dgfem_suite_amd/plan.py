@@ -56,6 +56,8 @@ class DGPlan:
             "face_nodes": np.ascontiguousarray(ref.face_nodes, dtype=np.int32),
             "geometry": np.ascontiguousarray(discr.packed_geometry(), dtype=np.float64),
             "ext_map": discr.packed_ext_map(),
+            "mass_inv": np.ascontiguousarray(ref.mass_inv, dtype=np.float64),
+            "face_mass": np.ascontiguousarray(ref.face_mass, dtype=np.float64),
         }
         h = self._host
         desc = _lib.DgPlanDesc()
@@ -68,6 +70,8 @@ class DGPlan:
         desc.face_nodes = _dptr(h["face_nodes"], ctypes.c_int32)
         desc.geometry = _dptr(h["geometry"], ctypes.c_double)
         desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
+        desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
+        desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
         desc.params[0] = self.param
         desc.variant = _lib.KERNEL_VARIANTS[variant]
         handle = ctypes.c_void_p()
@@ -76,6 +80,7 @@ class DGPlan:
                                                          ctypes.byref(handle)),
                        "dg_plan_create")
         self._handle = handle
+        self._vjp_workspace: list[torch.Tensor] | None = None
         del self._host  # the plan holds device copies
 
     # {{{ validation
@@ -174,6 +179,49 @@ class DGPlan:
             _lib.pointer_array([t.data_ptr() for t in res]),
             _lib.pointer_array([t.data_ptr() for t in rhs]),
             ctypes.c_void_p(stream)), "dg_lserk_stage")
+
+    def rhs_vjp(self, q: Sequence[torch.Tensor] | None, rhs_bar: Sequence[torch.Tensor],
+                out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:
+        """Vector-Jacobian product ``q_bar = (dL/dq)^T rhs_bar`` at state ``q``
+        (FP64).  ``q`` is read by Euler only (wave and Maxwell are linear) and
+        may be ``None`` for them.  Launches on the current stream; the
+        workspace is allocated on the first call and reused."""
+        if q is not None:
+            self._check_fields(q, "q")
+        elif self.equation == "euler":
+            raise ValueError("rhs_vjp: the Euler adjoint needs the state q")
+        self._check_fields(rhs_bar, "rhs_bar")
+        if out is None:
+            out = self.empty_fields()
+        else:
+            self._check_fields(out, "out")
+        if self._vjp_workspace is None:
+            self._vjp_workspace = self.empty_fields()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        pq = None if q is None else _lib.pointer_array([t.data_ptr() for t in q])
+        _lib.check(self.lib, self.lib.dg_rhs_vjp(
+            self._handle, pq,
+            _lib.pointer_array([t.data_ptr() for t in rhs_bar]),
+            _lib.pointer_array([t.data_ptr() for t in out]),
+            _lib.pointer_array([t.data_ptr() for t in self._vjp_workspace]),
+            ctypes.c_void_p(stream)), "dg_rhs_vjp")
+        return list(out)
+
+    def lserk_stage_adjoint(self, a: float, b: float, dt: float, q_bar: Sequence[torch.Tensor],
+                            s: Sequence[torch.Tensor],
+                            w: Sequence[torch.Tensor] | None) -> None:
+        """Reverse sweep of one LSERK4 stage: ``q_bar += dt*w; s = a*s + b*q_bar``."""
+        self._check_fields(q_bar, "q_bar")
+        self._check_fields(s, "s")
+        if w is not None:
+            self._check_fields(w, "w")
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_lserk_stage_adjoint(
+            self._handle, float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in q_bar]),
+            _lib.pointer_array([t.data_ptr() for t in s]),
+            None if w is None else _lib.pointer_array([t.data_ptr() for t in w]),
+            ctypes.c_void_p(stream)), "dg_lserk_stage_adjoint")
 
     def close(self) -> None:
         h = getattr(self, "_handle", None)

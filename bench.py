@@ -39,6 +39,8 @@ def _parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate CPU time budget of the cpu_baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--adjoint-steps", type=int, default=200,
+                   help="timed VJP evaluations for the adjoint/forward cost ratio (0: skip)")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run a plain device copy of known size (HBM counter "
                         "calibration under rocprofv3 --pmc)")
@@ -155,6 +157,29 @@ def main():
     wall = time.perf_counter() - t0
     kernel_ms = sum(a.elapsed_time(b) for a, b in zip(starts, ends)) / args.steps
 
+    adjoint = None
+    if not mixed and args.adjoint_steps > 0:
+        # VJP of the same RHS (the M^{-1} pass and the adjoint kernel), timed
+        # the same way; cotangent = the forward output
+        lam = [t.clone() for t in out]
+        qv = q if w.equation == "euler" else None
+        qbar = plan.empty_fields()
+        for _ in range(5):
+            plan.rhs_vjp(qv, lam, qbar)
+        torch.cuda.synchronize()
+        va = [torch.cuda.Event(enable_timing=True) for _ in range(args.adjoint_steps)]
+        vb = [torch.cuda.Event(enable_timing=True) for _ in range(args.adjoint_steps)]
+        for s in range(args.adjoint_steps):
+            va[s].record(stream)
+            plan.rhs_vjp(qv, lam, qbar)
+            vb[s].record(stream)
+        torch.cuda.synchronize()
+        vjp_ms = sum(a.elapsed_time(b) for a, b in zip(va, vb)) / args.adjoint_steps
+        adjoint = {"vjp_us": vjp_ms * 1e3, "fwd_us": kernel_ms * 1e3,
+                   "ratio": vjp_ms / kernel_ms, "steps": args.adjoint_steps,
+                   "kernels": "mass_inv_kernel + rhs_vjp_kernel"}
+        del lam, qbar
+
     if args.copy_probe:
         n = 256 * 1024 * 1024 // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
@@ -214,6 +239,8 @@ def main():
     }
     if field_err is not None:
         result["mixed_rel_l2_vs_fp64"] = field_err
+    if adjoint is not None:
+        result["adjoint"] = adjoint
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline sample ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)

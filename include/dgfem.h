@@ -84,6 +84,12 @@ typedef struct dg_plan_desc {
   double params[4];
   int32_t variant;           /* dg_kernel_variant */
   int32_t reserved;          /* must be 0 */
+  /* Needed by dg_rhs_vjp only (both NULL for a forward-only plan):
+   * (Np, Np) reference inverse mass M^{-1}, and (Np, 4*Nfp) reference face
+   * mass E with LIFT = M^{-1} E (mass_inverse_fuser.py:21-38 is the fusion
+   * the adjoint splits apart again). */
+  const double* mass_inv;
+  const double* face_mass;
 } dg_plan_desc;
 
 typedef struct dg_plan dg_plan;
@@ -123,7 +129,8 @@ int dg_rhs_mixed(const dg_plan* plan, int32_t precision,
 /* One stage of the five-stage, fourth-order 2N-storage Runge-Kutta scheme
  * (Carpenter & Kennedy 1994), as in libparanumal's LSERK4 that the reference
  * compares against (assets/h200nvl.patch:42):
- *   res = a*res + dt*rhs;  q = q + b*res   for every field.
+ *   res = a*res + dt*rhs;  q = q + b*res   for every field
+ * (a == 0, the first stage, does not read res).
  * The reference has no time stepper of its own. */
 int dg_lserk_stage(const dg_plan* plan, double a, double b, double dt,
                    double* const* q, double* const* res,
@@ -136,6 +143,29 @@ int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
                          double* const* q, double* const* res,
                          const float* const* rhs, float* const* q32,
                          void* stream);
+
+/* Vector-Jacobian product of the FP64 right-hand side at state q:
+ *   q_bar = (dL/dq)^T rhs_bar
+ * the exact transpose of dg_rhs for the linear wave and Maxwell operators
+ * (q may then be NULL) and of dg_rhs linearised at q for Euler.  `workspace`
+ * is nfields device buffers of (K, Np) float64 (overwritten; holds
+ * M^{-1} rhs_bar).  Two kernels are launched on `stream`.  The reference has
+ * no adjoint: its jax:jit array context (run/__main__.py:213) is where
+ * jax.vjp of the same RHS would come from.  Needs a plan created with
+ * mass_inv and face_mass, else DG_ERR_UNSUPPORTED. */
+int dg_rhs_vjp(const dg_plan* plan, const double* const* q,
+               const double* const* rhs_bar, double* const* q_bar,
+               double* const* workspace, void* stream);
+
+/* Reverse sweep of one LSERK4 stage (the transpose of dg_lserk_stage):
+ *   q_bar = q_bar + dt*w;   s = a*s + b*q_bar
+ * where w = dg_rhs_vjp(q_k, s) of the stage.  w may be NULL (the dt*w term is
+ * then skipped) and a == 0 does not read s, so the sweep of a step starts with
+ * (a=0, b=b_5, w=NULL) and runs stages k = 5..1 with (a_k, b_{k-1}, w_k).
+ * No counterpart in the reference (see dg_lserk_stage). */
+int dg_lserk_stage_adjoint(const dg_plan* plan, double a, double b, double dt,
+                           double* const* q_bar, double* const* s,
+                           const double* const* w, void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* dg_last_error_string(void);

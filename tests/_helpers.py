@@ -50,15 +50,16 @@ def interior_elements(od) -> np.ndarray:
     return np.nonzero(np.all(od.ext_index[:, :, 0] >= 0, axis=1))[0]
 
 
-def random_state(equation: str, x: np.ndarray, seed: int = 0) -> np.ndarray:
-    """A smooth state plus noise; Euler states stay physical."""
+def random_state(equation: str, x: np.ndarray, seed: int = 0, noise: float = 0.01) -> np.ndarray:
+    """A smooth state plus noise; Euler states stay physical.  ``noise``
+    scales the Euler state's per-node noise (0: continuous across faces)."""
     rng = np.random.default_rng(seed)
     shape = x.shape[1:]
     r2 = sum(x[i] ** 2 for i in range(3))
     if equation == "euler":
-        rho = 1.0 + 0.2 * np.exp(-r2) + 0.01 * rng.standard_normal(shape)
-        u = [0.3 * np.sin(x[0] + k) + 0.01 * rng.standard_normal(shape) for k in range(3)]
-        p = 1.0 + 0.1 * np.cos(x[1]) + 0.01 * rng.standard_normal(shape)
+        rho = 1.0 + 0.2 * np.exp(-r2) + noise * rng.standard_normal(shape)
+        u = [0.3 * np.sin(x[0] + k) + noise * rng.standard_normal(shape) for k in range(3)]
+        p = 1.0 + 0.1 * np.cos(x[1]) + noise * rng.standard_normal(shape)
         e = p / 0.4 + 0.5 * rho * sum(v * v for v in u)
         return np.stack([rho, e, *(rho * v for v in u)])
     nf = 4 if equation == "wave" else 6

@@ -54,6 +54,64 @@ def test_plan_create_rejects_bad_descriptors_without_touching_the_gpu():
     assert lib.dg_plan_destroy(None) == 0
 
 
+def _small_desc(keep):
+    """A valid descriptor of a 2^3 x 6 tet P1 mesh with the adjoint data;
+    ``keep`` holds the host arrays (private copies) alive."""
+    import numpy as np
+
+    from actx_dgfem_suite_amd.discretization import make_box_discretization
+
+    d = make_box_discretization(1, 2)
+    arrs = {
+        "d_weak": np.array(d.ref.d_weak, dtype=np.float64, order="C"),
+        "d_strong": np.array(d.ref.d_strong, dtype=np.float64, order="C"),
+        "lift": np.array(d.ref.lift, dtype=np.float64, order="C"),
+        "face_nodes": np.array(d.ref.face_nodes, dtype=np.int32, order="C"),
+        "geometry": np.array(d.packed_geometry(), dtype=np.float64, order="C"),
+        "ext_map": np.array(d.packed_ext_map(), dtype=np.int32, order="C"),
+        "mass_inv": np.array(d.ref.mass_inv, dtype=np.float64, order="C"),
+        "face_mass": np.array(d.ref.face_mass, dtype=np.float64, order="C"),
+    }
+    keep.update(arrs)
+    desc = _lib.DgPlanDesc()
+    desc.equation = 0
+    desc.order = 1
+    desc.nelements = d.nelements
+    for name, a in arrs.items():
+        ctype = ctypes.c_int32 if a.dtype == np.int32 else ctypes.c_double
+        setattr(desc, name, a.ctypes.data_as(ctypes.POINTER(ctype)))
+    desc.params[0] = 1.0
+    return desc
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libdgfem.so not built")
+def test_plan_create_checks_adjoint_data_without_touching_the_gpu():
+    """Adjoint inputs are validated on the host before any device call."""
+    lib = _lib.load_library()
+    handle = ctypes.c_void_p()
+    keep = {}
+    desc = _small_desc(keep)
+    desc.face_mass = None   # mass_inv without face_mass
+    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 1
+    assert b"together" in lib.dg_last_error_string()
+
+    desc = _small_desc(keep)
+    fm = keep["face_mass"]
+    fnodes = set(keep["face_nodes"][0].tolist())
+    off = next(i for i in range(fm.shape[0]) if i not in fnodes)
+    fm[off, 0] = 0.5        # couples a node off face 0
+    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 1
+    assert b"off its face" in lib.dg_last_error_string()
+
+    desc = _small_desc(keep)
+    em = keep["ext_map"]
+    row, col = next((r, c) for r in range(em.shape[0]) for c in range(em.shape[1])
+                    if em[r, c] >= 0 and c % 3 == 0)
+    em[row, col] = (em[row, col] + 4 * 7) % (em.shape[0] * 4)   # another element's node
+    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 1
+    assert b"ext_map" in lib.dg_last_error_string()
+
+
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.load_library.__wrapped__(str(tmp_path / "libdgfem.so"))

@@ -1,0 +1,341 @@
+"""GPU parity of the adjoint (dg_rhs_vjp, dg_lserk_stage_adjoint) and its
+autograd bindings.
+
+The reference has no adjoint of its own, and jax (whose ``jax.vjp`` of the
+reference's jax:jit RHS would be the natural cross-check) is not installed,
+so the adjoint is pinned by what the mathematics fixes:
+
+* the whole transpose, entry by entry: on a 48-tet mesh the dense operator
+  matrix of the oracle (linear equations) or its central-difference Jacobian
+  (Euler) against the matrix the VJP builds from unit cotangents;
+* the dot-product identity <L x, y> = <x, L^T y> against the HIP forward
+  kernel, at every order;
+* central finite differences of the Euler RHS and of whole LSERK4
+  trajectories (objectives quadratic in the final state), and the Maxwell
+  adjoint workload at its full size (16 directions, 1e-6).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from actx_dgfem_suite_amd import autodiff
+from actx_dgfem_suite_amd.plan import DGPlan
+from actx_dgfem_suite_amd.timestepping import LSERK4, cfl_timestep, trajectory_vjp
+from actx_dgfem_suite_amd.workloads import WORKLOADS, initial_state, maxwell_target
+from tests._helpers import discr_pair, oracle_rhs, random_state
+
+pytestmark = pytest.mark.gpu
+
+PARAM = {"wave": 1.0, "euler": 1.4, "maxwell": 0.5}
+NF = {"wave": 4, "euler": 5, "maxwell": 6}
+SPEED = {"wave": 1.0, "maxwell": 1.0, "euler": 2.0}
+
+
+def _dev(a, device):
+    return [torch.from_numpy(np.ascontiguousarray(f)).to(device) for f in a]
+
+
+def _np(fields):
+    return np.stack([t.cpu().numpy() for t in fields])
+
+
+def _dot(a, b) -> float:
+    return float(sum(torch.sum(x * y) for x, y in zip(a, b)))
+
+
+def _norm(a) -> float:
+    return float(torch.sqrt(sum(torch.sum(x * x) for x in a)))
+
+
+# {{{ the whole transpose on a tiny mesh
+
+def _dense_oracle(equation, od, q0, shape):
+    """Dense matrix of the oracle RHS (linear) or its central-difference
+    Jacobian at q0 (Euler), columns over the flattened state."""
+    n = int(np.prod(shape))
+    cols = []
+    for j in range(n):
+        e = np.zeros(n)
+        if equation == "euler":
+            eps = 1e-6 * max(1.0, abs(q0.ravel()[j]))
+            e[j] = eps
+            fp = oracle_rhs(equation, od, q0 + e.reshape(shape), PARAM[equation])
+            fm = oracle_rhs(equation, od, q0 - e.reshape(shape), PARAM[equation])
+            cols.append(((fp - fm) / (2 * eps)).ravel())
+        else:
+            e[j] = 1.0
+            cols.append(oracle_rhs(equation, od, e.reshape(shape), PARAM[equation]).ravel())
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("equation,order", [("wave", 1), ("wave", 2), ("maxwell", 1),
+                                            ("maxwell", 2), ("euler", 1)])
+def test_vjp_is_the_transpose(equation, order, device):
+    print(f"[case] dense transpose {equation} P{order} n=2", flush=True)
+    d, od = discr_pair(order, 2)
+    nf = NF[equation]
+    shape = (nf, d.nelements, d.np)
+    q0 = random_state(equation, od.x, seed=3)
+    a = _dense_oracle(equation, od, q0, shape)
+    plan = DGPlan(d, equation, PARAM[equation])
+    qd = _dev(q0, device) if equation == "euler" else None
+    n = a.shape[0]
+    at = np.empty((n, n))
+    lam = plan.empty_fields()
+    for i in range(n):
+        for t in lam:
+            t.zero_()
+        c, rest = divmod(i, d.nelements * d.np)
+        lam[c].view(-1)[rest] = 1.0
+        at[i] = _np(plan.rhs_vjp(qd, lam)).ravel()
+    torch.cuda.synchronize()
+    scale = np.abs(a).max()
+    err = np.abs(at - a).max() / scale
+    print(f"  max |VJP matrix - transpose| / max|L| = {err:.2e}", flush=True)
+    # Euler: the oracle side is a central difference (truncation ~1e-9)
+    assert err < (1e-7 if equation == "euler" else 1e-12)
+
+# }}}
+
+
+# {{{ dot-product identity against the forward kernel
+
+@pytest.mark.parametrize("equation", ["wave", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_dot_product_identity(equation, order, device):
+    print(f"[case] dot product {equation} P{order} n=4", flush=True)
+    d, od = discr_pair(order, 4)
+    plan = DGPlan(d, equation, PARAM[equation])
+    rng = np.random.default_rng(order)
+    shape = (NF[equation], d.nelements, d.np)
+    x = _dev(rng.standard_normal(shape), device)
+    y = _dev(rng.standard_normal(shape), device)
+    lx = plan.rhs(x)
+    lty = plan.rhs_vjp(None, y)
+    lhs, rhs = _dot(lx, y), _dot(x, lty)
+    rel = abs(lhs - rhs) / (_norm(lx) * _norm(y))
+    print(f"  <Lx,y>={lhs:.6e} <x,L^T y>={rhs:.6e} rel={rel:.2e}", flush=True)
+    assert rel < 1e-13
+
+
+def _lf_speed(q, gamma=1.4):
+    u2 = sum((q[2 + k] / q[0]) ** 2 for k in range(3))
+    p = (gamma - 1.0) * (q[1] - 0.5 * q[0] * u2)
+    return np.sqrt(u2) + np.sqrt(gamma * p / q[0])
+
+
+def _lf_branch_flips(d, q, dq) -> int:
+    """Interior face-node pairs whose max(lambda-, lambda+) changes branch
+    between q - dq and q + dq.  A central difference across that kink
+    averages two one-sided derivatives, so it cannot check a VJP there."""
+    em = d.packed_ext_map().reshape(d.nelements, -1)
+    own = np.arange(d.nelements)[:, None] * d.np + d.ref.face_nodes.reshape(-1)[None, :]
+    a, b = own[em >= 0], em[em >= 0]
+    lp, lm = _lf_speed(q + dq).ravel(), _lf_speed(q - dq).ravel()
+    return int(np.sum(np.sign(lp[a] - lp[b]) != np.sign(lm[a] - lm[b])))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_euler_vjp_central_difference(order, device):
+    """<(L(q+eps v) - L(q-eps v)) / 2 eps, y> = <v, J^T y> for the Euler
+    operator (truncation O(eps^2)), on the first seeded state whose
+    perturbation crosses no Lax-Friedrichs wavespeed kink."""
+    d, od = discr_pair(order, 4)
+    eps = 1e-5
+    for seed in range(order, order + 20):
+        q = random_state("euler", od.x, seed=seed)
+        rng = np.random.default_rng(10 + seed)
+        v = rng.standard_normal(q.shape) * np.abs(q).mean(axis=(1, 2), keepdims=True) * 0.1
+        if _lf_branch_flips(d, q, eps * v) == 0:
+            break
+    else:
+        pytest.fail("no kink-free state among 20 seeds")
+    print(f"[case] euler vjp fd P{order} n=4 seed={seed}", flush=True)
+    plan = DGPlan(d, "euler", PARAM["euler"])
+    y = _dev(rng.standard_normal(q.shape), device)
+    lp = plan.rhs(_dev(q + eps * v, device))
+    lm = plan.rhs(_dev(q - eps * v, device))
+    jv = [(a - b) / (2 * eps) for a, b in zip(lp, lm)]
+    jty = plan.rhs_vjp(_dev(q, device), y)
+    lhs, rhs = _dot(jv, y), _dot(_dev(v, device), jty)
+    rel = abs(lhs - rhs) / (_norm(jv) * _norm(y))
+    print(f"  <Jv,y>={lhs:.9e} <v,J^T y>={rhs:.9e} rel={rel:.2e}", flush=True)
+    assert rel < 1e-8
+
+
+def test_vjp_many_batches_and_workspace_reuse(device):
+    """A mesh where every persistent block runs several batches; two calls in
+    a row with different cotangents reuse the workspace."""
+    print("[case] dot product wave P3 n=12 (reuse)", flush=True)
+    d, od = discr_pair(3, 12)
+    plan = DGPlan(d, "wave", 1.0)
+    rng = np.random.default_rng(5)
+    shape = (4, d.nelements, d.np)
+    x = _dev(rng.standard_normal(shape), device)
+    lx = plan.rhs(x)
+    for seed in (1, 2):
+        y = _dev(np.random.default_rng(seed).standard_normal(shape), device)
+        rel = abs(_dot(lx, y) - _dot(x, plan.rhs_vjp(None, y))) / (_norm(lx) * _norm(y))
+        assert rel < 1e-13, (seed, rel)
+
+# }}}
+
+
+# {{{ autograd
+
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+def test_autograd_rhs(equation, device):
+    print(f"[case] autograd rhs {equation} P2 n=3", flush=True)
+    d, od = discr_pair(2, 3)
+    plan = DGPlan(d, equation, PARAM[equation])
+    q0 = random_state(equation, od.x, seed=1)
+    q = [t.requires_grad_() for t in _dev(q0, device)]
+    out = autodiff.rhs(plan, q)
+    for a, b in zip(out, plan.rhs(_dev(q0, device))):
+        assert torch.equal(a.detach(), b)
+    rng = np.random.default_rng(2)
+    w = _dev(rng.standard_normal((NF[equation], d.nelements, d.np)), device)
+    loss = sum(torch.sum(a * b) for a, b in zip(out, w))
+    grads = torch.autograd.grad(loss, q)
+    ref = plan.rhs_vjp(_dev(q0, device) if equation == "euler" else None, w)
+    for g, r in zip(grads, ref):
+        assert torch.equal(g, r)
+
+# }}}
+
+
+# {{{ trajectories
+
+def _objective(q, target):
+    return 0.5 * sum(torch.sum((a - b) ** 2) for a, b in zip(q, target))
+
+
+@pytest.mark.parametrize("equation", ["wave", "maxwell"])
+def test_trajectory_dot_product_and_checkpointing(equation, device):
+    """The step map is linear: <Phi x, y> = <x, Phi^T y>; and the gradient
+    does not depend on the checkpoint interval."""
+    print(f"[case] trajectory dot product {equation} P2 n=3", flush=True)
+    d, od = discr_pair(2, 3)
+    plan = DGPlan(d, equation, PARAM[equation])
+    st = LSERK4(plan, cfl_timestep(d, SPEED[equation]))
+    rng = np.random.default_rng(4)
+    shape = (NF[equation], d.nelements, d.np)
+    x = _dev(rng.standard_normal(shape), device)
+    y = _dev(rng.standard_normal(shape), device)
+    nsteps = 12
+    grads = {}
+    for every in (1, 5, 12):
+        phix, phity = trajectory_vjp(st, x, nsteps, y, checkpoint_every=every)
+        grads[every] = phity
+        rel = abs(_dot(phix, y) - _dot(x, phity)) / (_norm(phix) * _norm(y))
+        print(f"  every={every}: rel={rel:.2e}", flush=True)
+        assert rel < 1e-12
+    for every in (5, 12):
+        for a, b in zip(grads[every], grads[1]):
+            assert torch.allclose(a, b, rtol=0, atol=1e-13 * float(b.abs().max()))
+
+
+def test_euler_trajectory_central_difference(device):
+    """Gradient of J = 1/2 ||q(T) - q_tgt||^2 after 8 Euler LSERK4 steps
+    against central differences in 4 directions.  The perturbed trajectories
+    are checked to cross no Lax-Friedrichs wavespeed kink at any stage (a
+    continuous state would tie max(lambda-, lambda+) at every face)."""
+    print("[case] euler trajectory fd P2 n=3", flush=True)
+    d, od = discr_pair(2, 3)
+    plan = DGPlan(d, "euler", PARAM["euler"])
+    st = LSERK4(plan, cfl_timestep(d, SPEED["euler"]))
+    nsteps, eps = 8, 1e-6
+    q0 = random_state("euler", od.x, seed=6)
+    target = _dev(random_state("euler", od.x, seed=7), device)
+    q = [t.requires_grad_() for t in _dev(q0, device)]
+    qt = autodiff.trajectory(st, q, nsteps, checkpoint_every=3)
+    grads = torch.autograd.grad(_objective(qt, target), q)
+    rng = np.random.default_rng(8)
+
+    def stage_states(qs):
+        out = []
+        for _ in range(nsteps):
+            st.step(qs, out)
+        return qs, out
+
+    for k in range(4):
+        v = rng.standard_normal(q0.shape) * np.abs(q0).mean(axis=(1, 2), keepdims=True) * 0.05
+        qp, sp = stage_states(_dev(q0 + eps * v, device))
+        qm, sm = stage_states(_dev(q0 - eps * v, device))
+        flips = sum(_lf_branch_flips(d, 0.5 * (_np(a) + _np(b)), 0.5 * (_np(a) - _np(b)))
+                    for a, b in zip(sp, sm))
+        assert flips == 0, f"direction {k} crosses {flips} wavespeed kinks"
+        fd = float(_objective(qp, target) - _objective(qm, target)) / (2 * eps)
+        ad = _dot(grads, _dev(v, device))
+        print(f"  dir {k}: fd={fd:.10e} adjoint={ad:.10e} rel={abs(fd - ad) / abs(ad):.2e}",
+              flush=True)
+        assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def _run(st, q, nsteps):
+    st.run(q, nsteps)
+    return q
+
+
+def test_maxwell_adjoint_workload_gradient(device):
+    """The Maxwell adjoint workload at its full size: 25^3 x 6 tets, P4,
+    100 LSERK4 steps at the CFL step, checkpoints every 10, objective
+    J = 1/2 ||E(T) - E_tgt||^2; the gradient w.r.t. the initial state
+    against central differences on 16 random directions (seed 2)."""
+    w = WORKLOADS["maxwell_p4"]
+    print(f"[case] maxwell adjoint workload K={6 * w.nel_1d ** 3} P4 100 steps", flush=True)
+    d = w.discretization()
+    plan = DGPlan(d, "maxwell", w.param)
+    st = LSERK4(plan, cfl_timestep(d, 1.0))
+    q0 = initial_state(w, d)
+    e_tgt = _dev(maxwell_target(d)[:3], device)
+
+    def objective(q):
+        return 0.5 * sum(torch.sum((a - b) ** 2) for a, b in zip(q[:3], e_tgt))
+
+    q = [t.requires_grad_() for t in _dev(q0, device)]
+    qt = autodiff.trajectory(st, q, 100, checkpoint_every=10)
+    j0 = objective(qt)
+    grads = torch.autograd.grad(j0, q)
+    print(f"  J={float(j0):.10e}", flush=True)
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for k in range(16):
+        v = rng.standard_normal(q0.shape)
+        eps = 1e-3
+        jp = objective(_run(st, _dev(q0 + eps * v, device), 100))
+        jm = objective(_run(st, _dev(q0 - eps * v, device), 100))
+        fd = float(jp - jm) / (2 * eps)
+        ad = _dot(grads, _dev(v, device))
+        rel = abs(fd - ad) / abs(ad)
+        worst = max(worst, rel)
+        print(f"  dir {k:2d}: fd={fd:.12e} adjoint={ad:.12e} rel={rel:.2e}", flush=True)
+    assert worst < 1e-6
+
+# }}}
+
+
+@pytest.mark.parametrize("equation", ["wave", "maxwell"])
+def test_upwind_energy_does_not_grow(equation, device):
+    """Upwind fluxes dissipate: the discrete energy sum_e J_e q^T M q stays
+    below its initial value along an LSERK4 trajectory at the CFL step."""
+    print(f"[case] energy {equation} P3 n=4 60 steps", flush=True)
+    d, od = discr_pair(3, 4)
+    plan = DGPlan(d, equation, 1.0)
+    st = LSERK4(plan, cfl_timestep(d, 1.0))
+    q = _dev(random_state(equation, od.x, seed=9), device)
+    mass = torch.from_numpy(d.ref.mass).to(device)
+    jac = torch.from_numpy(np.ascontiguousarray(d.geometry.jac)).to(device)
+
+    def energy():
+        return float(sum(torch.sum(jac[:, None] * (f @ mass) * f) for f in q))
+
+    e0 = energy()
+    for _ in range(60):
+        st.step(q)
+        cur = energy()
+        # one RK step of a dissipative but non-normal operator may not be
+        # strictly contractive; the energy never exceeds its initial value
+        assert cur <= e0 * (1 + 1e-12), (e0, cur)
+    assert cur < e0
