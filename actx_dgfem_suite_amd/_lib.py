@@ -53,6 +53,7 @@ class DgPlanDesc(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("mass_inv", ctypes.POINTER(ctypes.c_double)),
         ("face_mass", ctypes.POINTER(ctypes.c_double)),
+        ("centroids", ctypes.POINTER(ctypes.c_double)),
     ]
 
 

@@ -60,15 +60,24 @@ class _Queue:
 class HIPDGFEMArrayContext:
     """Array context backed by hand-written gfx950 kernels.
 
-    ``precision``: ``"fp64"`` (the drop-in form).
+    ``precision``: ``"fp64"`` (the drop-in form), or ``"fp32"`` / ``"bf16x3"``
+    (the mixed form: state leaves are float32 tensors, ``from_numpy``
+    converts).  ``kernel_variant`` picks the FP64 kernel (``"auto"``,
+    ``"mfma"``, ``"lds"``).  ``verify_rtol``: the tolerance the runner's value
+    check applies to this context.
     """
 
-    def __init__(self, device: torch.device | int | None = None, precision: str = "fp64"):
+    kernel_variant = "auto"
+    default_precision = "fp64"
+    verify_rtol = 1e-5
+
+    def __init__(self, device: torch.device | int | None = None,
+                 precision: str | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("HIPDGFEMArrayContext needs a ROCm GPU (no CPU fallback)")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None
                                    else torch.device(device).index or 0)
-        self.precision = precision
+        self.precision = self.default_precision if precision is None else precision
         self.queue = _Queue(self.device)
         self._plans: dict[int, Any] = {}
         self.np = SimpleNamespace(
@@ -80,9 +89,18 @@ class HIPDGFEMArrayContext:
 
     # {{{ data movement
 
+    @property
+    def state_dtype(self) -> torch.dtype:
+        return torch.float64 if self.precision == "fp64" else torch.float32
+
     def from_numpy(self, x):
-        return map_leaves(lambda a: torch.as_tensor(np.ascontiguousarray(a),
-                                                     device=self.device), x)
+        dtype = self.state_dtype
+
+        def leaf(a):
+            t = torch.as_tensor(np.ascontiguousarray(a), device=self.device)
+            return t.to(dtype) if t.is_floating_point() else t
+
+        return map_leaves(leaf, x)
 
     def to_numpy(self, x):
         return map_leaves(lambda t: t.detach().cpu().numpy(), x)
@@ -109,7 +127,8 @@ class HIPDGFEMArrayContext:
         entry = self._plans.get(key)
         if entry is None:
             from actx_dgfem_suite_amd.plan import DGPlan
-            entry = (op, DGPlan(op.discr, op.equation, op.param, device=self.device))
+            entry = (op, DGPlan(op.discr, op.equation, op.param, device=self.device,
+                                variant=self.kernel_variant))
             self._plans[key] = entry
         return entry[1]
 
@@ -144,3 +163,20 @@ class HIPDGFEMArrayContext:
 
     def clone(self) -> "HIPDGFEMArrayContext":
         return type(self)(device=self.device, precision=self.precision)
+
+
+class HIPDGFEMLDSArrayContext(HIPDGFEMArrayContext):
+    """FP64 drop-in form on the LDS-staged baseline kernel."""
+    kernel_variant = "lds"
+
+
+class HIPDGFEMFP32ArrayContext(HIPDGFEMArrayContext):
+    """Mixed form: FP32 state, FP32 MFMA contractions, FP64 geometry and flux.
+    Its value check allows the FP32 state's rounding (relative 1e-3)."""
+    default_precision = "fp32"
+    verify_rtol = 1e-3
+
+
+class HIPDGFEMBF16x3ArrayContext(HIPDGFEMFP32ArrayContext):
+    """Mixed form with the contractions as a three-product bf16 split."""
+    default_precision = "bf16x3"

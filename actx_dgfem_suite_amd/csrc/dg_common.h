@@ -22,7 +22,21 @@ constexpr int nfields_of(int eq) { return eq == DG_EQ_WAVE ? 4 : (eq == DG_EQ_EU
 struct InPtrs { const double* p[kMaxFields]; };
 struct OutPtrs { double* p[kMaxFields]; };
 
+// Work schedule of a kernel whose units are runs of consecutive elements
+// (tiles or batches): `order` lists the units in processing order and
+// `part[x] .. part[x+1]` is the slice of `order` that XCD x works through.
+// Workgroups are dispatched round-robin over the 8 XCDs, so XCD x runs blocks
+// x, x + 8, ...; all workers of one XCD step through their slice together
+// (worker s takes positions s, s + nworkers, ...), which keeps the units in
+// flight on an XCD -- and the neighbours their exterior gathers touch --
+// inside a compact region of the mesh that fits that XCD's L2.
+struct Schedule {
+  const int32_t* order;
+  const int32_t* part;   // 9 offsets
+};
+
 struct RhsArgs {
+  Schedule sched;
   InPtrs q;
   OutPtrs rhs;
   const double* geo;      // (K, 25)
@@ -36,13 +50,11 @@ struct RhsArgs {
   double p0;
 };
 
-// XCD-aware work split: blocks are dealt round-robin to the 8 XCDs, so the
-// logical block id gives each XCD one contiguous run of work (neighbour
-// gathers then stay in that XCD's L2).  Needs gridDim.x % 8 == 0.
-__device__ inline int logical_block() {
-  const int xcd_blocks = gridDim.x / 8;
-  return (blockIdx.x % 8) * xcd_blocks + blockIdx.x / 8;
-}
+// This block's XCD and its index among the blocks of that XCD.  Needs
+// gridDim.x % 8 == 0.
+__device__ inline int block_xcd() { return blockIdx.x & 7; }
+__device__ inline int block_in_xcd() { return blockIdx.x >> 3; }
+__device__ inline int blocks_per_xcd() { return gridDim.x >> 3; }
 
 // {{{ numerical fluxes: out[c] is the integrand of +face_mass for field c
 

@@ -18,8 +18,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -175,6 +178,90 @@ MixedKernel pick_mixed(int prec, int eq, int order) {
   if (prec == DG_PREC_BF16X3) return pick_mixed_prec<DG_PREC_BF16X3>(eq, order);
   return nullptr;
 }
+
+// {{{ locality schedule
+
+// Spreads the bits of a 10-bit integer three apart (Morton interleave).
+uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Work order for units of `unit` consecutive elements (see Schedule in
+// dg_common.h).  With element centroids, the units are split into 8 parts of
+// equal count by recursive coordinate bisection (longest axis at the median)
+// and each part is ordered along a Morton curve of the unit centroids: every
+// XCD works on a compact eighth of the mesh and sweeps it so that the units
+// in flight, and their face neighbours, stay close.  Without centroids the
+// units keep their storage order and the parts are 8 contiguous runs.
+void build_schedule(int64_t nelem, int unit, const double* centroids,
+                    std::vector<int32_t>& order, std::array<int32_t, 9>& part) {
+  const int64_t nunits = (nelem + unit - 1) / unit;
+  order.resize(nunits);
+  std::iota(order.begin(), order.end(), 0);
+  for (int x = 0; x <= 8; ++x) part[x] = (int32_t)(nunits * x / 8);
+  if (!centroids) return;
+  std::vector<std::array<double, 3>> c(nunits, {0.0, 0.0, 0.0});
+  for (int64_t u = 0; u < nunits; ++u) {
+    const int64_t e0 = u * unit, e1 = std::min<int64_t>(nelem, e0 + unit);
+    for (int64_t e = e0; e < e1; ++e)
+      for (int d = 0; d < 3; ++d) c[u][d] += centroids[e * 3 + d];
+    for (int d = 0; d < 3; ++d) c[u][d] /= (double)(e1 - e0);
+  }
+  auto bbox = [&](int64_t b, int64_t e, double* lo, double* hi) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = 1e300;
+      hi[d] = -1e300;
+    }
+    for (int64_t k = b; k < e; ++k)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], c[order[k]][d]);
+        hi[d] = std::max(hi[d], c[order[k]][d]);
+      }
+  };
+  // bisection into parts [part[x], part[x+1]) of 8 equal counts
+  struct Range { int64_t b, e; int p0, np; };
+  std::vector<Range> stack{{0, nunits, 0, 8}};
+  while (!stack.empty()) {
+    const Range r = stack.back();
+    stack.pop_back();
+    double lo[3], hi[3];
+    bbox(r.b, r.e, lo, hi);
+    if (r.np == 1) {
+      std::vector<std::pair<uint32_t, int32_t>> key(r.e - r.b);
+      for (int64_t k = r.b; k < r.e; ++k) {
+        uint32_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+          const double w = hi[d] - lo[d];
+          const double t = w > 0 ? (c[order[k]][d] - lo[d]) / w : 0.0;
+          code |= spread3((uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0))) << d;
+        }
+        key[k - r.b] = {code, order[k]};
+      }
+      std::sort(key.begin(), key.end());
+      for (int64_t k = r.b; k < r.e; ++k) order[k] = key[k - r.b].second;
+      continue;
+    }
+    int axis = 0;
+    for (int d = 1; d < 3; ++d)
+      if (hi[d] - lo[d] > hi[axis] - lo[axis]) axis = d;
+    const int half = r.np / 2;
+    const int64_t mid = part[r.p0 + half];
+    if (mid > r.b && mid < r.e)
+      std::nth_element(order.begin() + r.b, order.begin() + mid, order.begin() + r.e,
+                       [&](int32_t x, int32_t y) {
+                         return c[x][axis] < c[y][axis] || (c[x][axis] == c[y][axis] && x < y);
+                       });
+    stack.push_back({r.b, mid, r.p0, half});
+    stack.push_back({mid, r.e, r.p0 + half, r.np - half});
+  }
+}
+
+// }}}
 
 // {{{ adjoint kernels and their host-side data
 
@@ -411,6 +498,12 @@ struct dg_plan {
   uint32_t* mixed_lift[2] = {nullptr, nullptr};
   MixedKernel mixed_kernel[2] = {nullptr, nullptr};
   int mixed_grid[2] = {0, 0};
+  // locality schedules: 16-element tiles (MFMA and mixed kernels) and
+  // batches of kBlock / Np elements (LDS-staged and adjoint kernels)
+  int32_t* sched_tile = nullptr;
+  int32_t* part_tile = nullptr;
+  int32_t* sched_batch = nullptr;
+  int32_t* part_batch = nullptr;
   // adjoint (present when the descriptor carried mass_inv and face_mass)
   bool has_vjp = false;
   VjpKernel vjp_kernel = nullptr;
@@ -439,6 +532,10 @@ void free_plan(dg_plan* p) {
     (void)hipFree(p->mixed_op[k]);
     (void)hipFree(p->mixed_lift[k]);
   }
+  (void)hipFree(p->sched_tile);
+  (void)hipFree(p->part_tile);
+  (void)hipFree(p->sched_batch);
+  (void)hipFree(p->part_batch);
   (void)hipFree(p->dmat);
   (void)hipFree(p->face_mass);
   (void)hipFree(p->mass_inv);
@@ -488,6 +585,7 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
     a.q[c] = in[c];
     a.rhs[c] = out[c];
   }
+  a.sched = {p->sched_tile, p->part_tile};
   a.geo = p->geo;
   a.emap = p->emap;
   a.fmask = p->fmask;
@@ -608,6 +706,22 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     free_plan(p);
     return rc;
   }
+  {
+    std::vector<int32_t> order;
+    std::array<int32_t, 9> part;
+    build_schedule(k, 16, d->centroids, order, part);
+    if ((rc = upload(&p->sched_tile, order.data(), order.size())) ||
+        (rc = upload(&p->part_tile, part.data(), part.size()))) {
+      free_plan(p);
+      return rc;
+    }
+    build_schedule(k, kBlock / np, d->centroids, order, part);
+    if ((rc = upload(&p->sched_batch, order.data(), order.size())) ||
+        (rc = upload(&p->part_batch, part.data(), part.size()))) {
+      free_plan(p);
+      return rc;
+    }
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) {
     free_plan(p);
@@ -703,6 +817,8 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
     a.q.p[c] = in[c];
     a.rhs.p[c] = out[c];
   }
+  a.sched = p->variant == DG_KERNEL_LDS ? Schedule{p->sched_batch, p->part_batch}
+                                        : Schedule{p->sched_tile, p->part_tile};
   a.geo = p->geo;
   a.emap = p->emap;
   a.dmat_t = p->dmat_t;
@@ -767,6 +883,7 @@ int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rh
   hipLaunchKernelGGL(p->massinv_kernel, dim3(p->massinv_grid), dim3(kBlock), 0,
                      (hipStream_t)stream, a.lam, mu, p->nfields, p->mass_inv, p->nelem);
   DG_HIP_CHECK(hipGetLastError());
+  a.sched = {p->sched_batch, p->part_batch};
   a.geo = p->geo;
   a.emap = p->emap;
   a.fmask = p->fmask;

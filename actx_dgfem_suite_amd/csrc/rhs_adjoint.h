@@ -30,6 +30,7 @@
 namespace dg {
 
 struct VjpArgs {
+  Schedule sched;    // batches of kBlock / Np elements
   InPtrs q;          // state (Euler only; may be NULL for wave / Maxwell)
   InPtrs lam;        // cotangent of the RHS
   InPtrs mu;         // M^{-1} lam per element (workspace)
@@ -239,15 +240,14 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
   for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
   for (int k = tid; k < NP * 3; k += kBlock) s_nf[k] = a.node_faces[k];
 
-  const int64_t nbatch = (a.nelem + EB - 1) / EB;
-  const int64_t per = (nbatch + gridDim.x - 1) / gridDim.x;
-  const int64_t b_begin = (int64_t)logical_block() * per;
-  const int64_t b_end = b_begin + per < nbatch ? b_begin + per : nbatch;
+  const int32_t p_end = a.sched.part[block_xcd() + 1];
   const int el = tid / NP;
   const int i = tid - el * NP;
   const double c0 = a.p0;
 
-  for (int64_t b = b_begin; b < b_end; ++b) {
+  for (int32_t pos = a.sched.part[block_xcd()] + block_in_xcd(); pos < p_end;
+       pos += blocks_per_xcd()) {
+    const int64_t b = a.sched.order[pos];
     const int64_t e0 = b * EB;
     const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
     __syncthreads();

@@ -1,10 +1,11 @@
 // Fused FP64 RHS kernel on the matrix cores (v_mfma_f64_16x16x4f64).
 //
-// Each wave owns a contiguous run of tiles of 16 consecutive elements (the N
-// dimension of the MFMA) and works on them independently of the other waves
-// of its block -- the only block-wide barrier is the initial load of the
-// constant operators into LDS -- so the memory, LDS, VALU and MFMA phases of
-// different waves overlap freely.
+// A tile is 16 consecutive elements (the N dimension of the MFMA).  Each wave
+// takes tiles from its XCD's slice of the plan's locality schedule and works
+// on them independently of the other waves of its block -- the only
+// block-wide barrier is the initial load of the constant operators into LDS
+// -- so the memory, LDS, VALU and MFMA phases of different waves overlap
+// freely.
 //
 // Per tile:
 //   volume: B operands straight from HBM into registers (lane group g loads a
@@ -120,17 +121,16 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   const int lane = tid & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
-  // contiguous tile runs per wave; waves of a block are independent
-  const int64_t ntiles = (a.nelem + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * WAVES;
-  const int64_t wid = (int64_t)logical_block() * WAVES + (tid >> 6);
-  const int64_t per = (ntiles + nwaves - 1) / nwaves;
-  const int64_t t_begin = wid * per;
-  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  // every wave is a worker of its XCD's slice of the tile schedule; waves of
+  // a block are independent
+  const int slot = block_in_xcd() * WAVES + (tid >> 6);
+  const int nslots = blocks_per_xcd() * WAVES;
+  const int32_t p_end = a.sched.part[block_xcd() + 1];
   const double c0 = a.p0;
   double* const buf = s_buf[tid >> 6];
 
-  for (int64_t tile = t_begin; tile < t_end; ++tile) {
+  for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
+    const int64_t tile = a.sched.order[pos];
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads

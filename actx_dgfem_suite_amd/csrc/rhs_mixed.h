@@ -32,6 +32,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 
 struct MixedArgs {
+  Schedule sched;           // 16-element tiles
   const float* q[kMaxFields];
   float* rhs[kMaxFields];
   const double* geo;        // (K, 25), FP64
@@ -143,12 +144,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   const int lane = tid & 63;
   const int g = lane >> 4;
   const int n = lane & 15;
-  const int64_t ntiles = (a.nelem + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * WAVES;
-  const int64_t wid = (int64_t)logical_block() * WAVES + (tid >> 6);
-  const int64_t per = (ntiles + nwaves - 1) / nwaves;
-  const int64_t t_begin = wid * per;
-  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  const int slot = block_in_xcd() * WAVES + (tid >> 6);
+  const int nslots = blocks_per_xcd() * WAVES;
+  const int32_t p_end = a.sched.part[block_xcd() + 1];
   const double c0 = a.p0;
   float* const buf = s_buf[tid >> 6];
 
@@ -159,7 +157,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     return &s_lift[(((b * (KSL / KG)) + kq) * 64 + lane) * AW];
   };
 
-  for (int64_t tile = t_begin; tile < t_end; ++tile) {
+  for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
+    const int64_t tile = a.sched.order[pos];
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads

@@ -58,6 +58,8 @@ class DGPlan:
             "ext_map": discr.packed_ext_map(),
             "mass_inv": np.ascontiguousarray(ref.mass_inv, dtype=np.float64),
             "face_mass": np.ascontiguousarray(ref.face_mass, dtype=np.float64),
+            # element centroids: the plan's locality schedule
+            "centroids": np.ascontiguousarray(discr.nodes.mean(axis=2).T, dtype=np.float64),
         }
         h = self._host
         desc = _lib.DgPlanDesc()
@@ -72,6 +74,7 @@ class DGPlan:
         desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
         desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
         desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
+        desc.centroids = _dptr(h["centroids"], ctypes.c_double)
         desc.params[0] = self.param
         desc.variant = _lib.KERNEL_VARIANTS[variant]
         handle = ctypes.c_void_p()

@@ -10,14 +10,23 @@ NDOFS_TINY = 1_000
 NDOFS_LARGE = 20_000_000
 
 
-def get_rhs(equation: str, actx, dim: int, degree: int) -> tuple[Any, tuple[Any, ...]]:
-    """Return ``(rhs_callable, call_args)`` for ``equation``."""
+def parse_equation(equation: str) -> tuple[int, str]:
+    """``(ndofs per field, base equation)`` of a tiered name such as
+    ``tiny_wave`` (rhs_builder.py:14-29)."""
     if equation.startswith("tiny_"):
         ndofs, base = NDOFS_TINY, equation.removeprefix("tiny_")
     elif equation.startswith("large_"):
         ndofs, base = NDOFS_LARGE, equation.removeprefix("large_")
     else:
         ndofs, base = NDOFS_FULL, equation
+    if base not in ("wave", "euler", "maxwell"):
+        raise NotImplementedError(f"equation '{equation}' not supported")
+    return ndofs, base
+
+
+def get_rhs(equation: str, actx, dim: int, degree: int) -> tuple[Any, tuple[Any, ...]]:
+    """Return ``(rhs_callable, call_args)`` for ``equation``."""
+    ndofs, base = parse_equation(equation)
     if base == "wave":
         from actx_dgfem_suite_amd.equations.wave import get_wave_rhs
         return get_wave_rhs(actx=actx, dim=dim, order=degree, ndofs=ndofs)

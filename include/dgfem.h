@@ -90,6 +90,10 @@ typedef struct dg_plan_desc {
    * the adjoint splits apart again). */
   const double* mass_inv;
   const double* face_mass;
+  /* Optional (NULL: storage order): (K, 3) element centroids.  The plan
+   * uses them only to order its work for cache locality (each XCD sweeps a
+   * compact part of the mesh); results do not depend on them. */
+  const double* centroids;
 } dg_plan_desc;
 
 typedef struct dg_plan dg_plan;
