@@ -27,12 +27,15 @@ EXPORTED_SYMBOLS = (
     "dg_plan_create",
     "dg_plan_destroy",
     "dg_plan_info",
+    "dg_plan_variant",
     "dg_rhs",
     "dg_rhs_mixed",
     "dg_lserk_stage",
     "dg_lserk_stage_mixed",
     "dg_rhs_vjp",
     "dg_lserk_stage_adjoint",
+    "dg_rhs_vjp_fp32",
+    "dg_lserk_stage_adjoint_fp32",
     "dg_last_error_string",
 )
 
@@ -77,6 +80,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                  ctypes.POINTER(ctypes.c_int32),
                                  ctypes.POINTER(ctypes.c_int64)]
     lib.dg_plan_info.restype = ctypes.c_int
+    lib.dg_plan_variant.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    lib.dg_plan_variant.restype = ctypes.c_int
     lib.dg_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int32, _PP, _PP, ctypes.c_void_p]
     lib.dg_rhs.restype = ctypes.c_int
     lib.dg_lserk_stage.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
@@ -92,6 +97,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_lserk_stage_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, _PP, _PP, _PP, ctypes.c_void_p]
     lib.dg_lserk_stage_adjoint.restype = ctypes.c_int
+    lib.dg_rhs_vjp_fp32.argtypes = [ctypes.c_void_p, _PP, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_rhs_vjp_fp32.restype = ctypes.c_int
+    lib.dg_lserk_stage_adjoint_fp32.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, _PP, _PP,
+                                                _PP, ctypes.c_void_p]
+    lib.dg_lserk_stage_adjoint_fp32.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []
     lib.dg_last_error_string.restype = ctypes.c_char_p
     return lib

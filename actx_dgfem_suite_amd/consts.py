@@ -14,12 +14,19 @@ DEV_TO_PEAK_BW = {
 }
 
 
-def lookup_device(name: str) -> str | None:
+# ISA name -> table key, for runtimes that report a generic device name
+ARCH_TO_DEVICE = {
+    "gfx950": "AMD Instinct MI355X",
+}
+
+
+def lookup_device(name: str, arch: str = "") -> str | None:
     """The table key for a runtime device name (exact, else by substring:
-    ROCm reports e.g. ``"AMD Instinct MI355X"`` or a longer marketing name)."""
+    ROCm reports e.g. ``"AMD Instinct MI355X"`` or a longer marketing name),
+    else by the ISA name (``gcnArchName``, e.g. ``"gfx950:sramecc+:xnack-"``)."""
     if name in DEV_TO_PEAK_BW:
         return name
     for key in DEV_TO_PEAK_BW:
         if key.split()[-1] in name:
             return key
-    return None
+    return ARCH_TO_DEVICE.get(arch.split(":")[0])

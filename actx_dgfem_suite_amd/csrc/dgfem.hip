@@ -79,29 +79,32 @@ __global__ __launch_bounds__(kBlock) void lserk_stage_kernel(LserkArgs s) {
 }
 
 // Reverse sweep of one stage: q_bar += dt*w (w optional); s = a*s + b*q_bar
-// (s not read when a == 0).
+// (s not read when a == 0).  float64, or float32 for the FP32 adjoint.
+template <typename T>
 struct LserkAdjArgs {
-  OutPtrs qbar;
-  OutPtrs s;
-  InPtrs w;
+  T* qbar[kMaxFields];
+  T* s[kMaxFields];
+  const T* w[kMaxFields];
   int64_t n;
   int nf;
   double a, b, dt;
 };
 
-__global__ __launch_bounds__(kBlock) void lserk_adjoint_kernel(LserkAdjArgs s) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lserk_adjoint_kernel(LserkAdjArgs<T> s) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const T a = (T)s.a, b = (T)s.b, dt = (T)s.dt;
   for (int f = 0; f < s.nf; ++f) {
-    double* qb = s.qbar.p[f];
-    double* sv = s.s.p[f];
-    const double* w = s.w.p[f];
+    T* qb = s.qbar[f];
+    T* sv = s.s[f];
+    const T* w = s.w[f];
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < s.n; k += stride) {
-      double q = qb[k];
+      T q = qb[k];
       if (w) {
-        q += s.dt * w[k];
+        q += dt * w[k];
         qb[k] = q;
       }
-      sv[k] = (s.a != 0.0 ? s.a * sv[k] : 0.0) + s.b * q;
+      sv[k] = (s.a != 0.0 ? a * sv[k] : (T)0) + b * q;
     }
   }
 }
@@ -177,6 +180,15 @@ MixedKernel pick_mixed(int prec, int eq, int order) {
   if (prec == DG_PREC_FP32) return pick_mixed_prec<DG_PREC_FP32>(eq, order);
   if (prec == DG_PREC_BF16X3) return pick_mixed_prec<DG_PREC_BF16X3>(eq, order);
   return nullptr;
+}
+
+// The FP64 kernel "auto" picks, from measurements on MI355X at 1M-4M DOFs per
+// field: the MFMA kernel, except for Maxwell at P2-P4, whose 18
+// reference-derivative contractions per element make the MFMA kernel
+// register-bound (one wave per SIMD, spills at P4) while the LDS-staged
+// kernel keeps 2-3 waves per SIMD.
+int auto_variant(int equation, int order) {
+  return (equation == DG_EQ_MAXWELL && order >= 2) ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
 }
 
 // {{{ locality schedule
@@ -265,37 +277,74 @@ void build_schedule(int64_t nelem, int unit, const double* centroids,
 
 // {{{ adjoint kernels and their host-side data
 
-using VjpKernel = void (*)(VjpArgs);
-using MassInvKernel = void (*)(InPtrs, OutPtrs, int, const double*, int64_t);
+template <typename T>
+using VjpKernel = void (*)(VjpArgs<T>);
+template <typename T>
+using MassInvKernel = void (*)(MassInvArgs<T>);
 
-template <int EQ>
-VjpKernel pick_vjp_order(int order) {
+template <int EQ, typename T>
+VjpKernel<T> pick_vjp_order(int order) {
   switch (order) {
-    case 1: return rhs_vjp_kernel<EQ, 1>;
-    case 2: return rhs_vjp_kernel<EQ, 2>;
-    case 3: return rhs_vjp_kernel<EQ, 3>;
-    case 4: return rhs_vjp_kernel<EQ, 4>;
+    case 1: return rhs_vjp_kernel<EQ, 1, T>;
+    case 2: return rhs_vjp_kernel<EQ, 2, T>;
+    case 3: return rhs_vjp_kernel<EQ, 3, T>;
+    case 4: return rhs_vjp_kernel<EQ, 4, T>;
     default: return nullptr;
   }
 }
 
-VjpKernel pick_vjp(int eq, int order) {
+template <typename T>
+VjpKernel<T> pick_vjp(int eq, int order) {
   switch (eq) {
-    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE>(order);
-    case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER>(order);
-    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL>(order);
+    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE, T>(order);
+    case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER, T>(order);
+    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL, T>(order);
     default: return nullptr;
   }
 }
 
-MassInvKernel pick_mass_inv(int order) {
+template <typename T>
+MassInvKernel<T> pick_mass_inv(int order) {
   switch (order) {
-    case 1: return mass_inv_kernel<1>;
-    case 2: return mass_inv_kernel<2>;
-    case 3: return mass_inv_kernel<3>;
-    case 4: return mass_inv_kernel<4>;
+    case 1: return mass_inv_kernel<1, T>;
+    case 2: return mass_inv_kernel<2, T>;
+    case 3: return mass_inv_kernel<3, T>;
+    case 4: return mass_inv_kernel<4, T>;
     default: return nullptr;
   }
+}
+
+// Inverse of a small dense matrix (Gauss-Jordan, partial pivoting); false if
+// singular.
+bool invert(int n, const double* a, std::vector<double>& inv) {
+  std::vector<double> m(a, a + (size_t)n * n);
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(m[(size_t)r * n + c]) > std::fabs(m[(size_t)piv * n + c])) piv = r;
+    if (m[(size_t)piv * n + c] == 0.0) return false;
+    for (int k = 0; k < n; ++k) {
+      std::swap(m[(size_t)c * n + k], m[(size_t)piv * n + k]);
+      std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+    }
+    const double d = 1.0 / m[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      m[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = m[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        m[(size_t)r * n + k] -= f * m[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  return true;
 }
 
 // Host data of the adjoint: per-face blocks of the face mass, the face nodes
@@ -506,11 +555,14 @@ struct dg_plan {
   int32_t* part_batch = nullptr;
   // adjoint (present when the descriptor carried mass_inv and face_mass)
   bool has_vjp = false;
-  VjpKernel vjp_kernel = nullptr;
-  MassInvKernel massinv_kernel = nullptr;
+  VjpKernel<double> vjp_kernel = nullptr;
+  VjpKernel<float> vjp_kernel32 = nullptr;
+  MassInvKernel<double> massinv_kernel = nullptr;
+  MassInvKernel<float> massinv_kernel32 = nullptr;
   int vjp_grid = 0;
+  int vjp_grid32 = 0;
   int massinv_grid = 0;
-  double* dmat = nullptr;        // (3, Np, Np) untransposed D
+  double* amat_t = nullptr;      // (3, Np, Np): A_r = D_r^T M at [r][j][i]
   double* face_mass = nullptr;   // (4, Nfp, Nfp)
   double* mass_inv = nullptr;    // (Np, Np)
   double* ext_fscale = nullptr;  // (K, 4)
@@ -536,7 +588,7 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->part_tile);
   (void)hipFree(p->sched_batch);
   (void)hipFree(p->part_batch);
-  (void)hipFree(p->dmat);
+  (void)hipFree(p->amat_t);
   (void)hipFree(p->face_mass);
   (void)hipFree(p->mass_inv);
   (void)hipFree(p->ext_fscale);
@@ -659,13 +711,27 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       return fail(DG_ERR_INVALID_ARGUMENT, "face_nodes entry out of range");
   // adjoint data, built and checked before anything touches the device
   const bool want_vjp = d->mass_inv || d->face_mass;
-  std::vector<double> fm, xfs;
+  std::vector<double> fm, xfs, amat_t;
   std::vector<int32_t> nf;
   if (want_vjp) {
     if (!d->mass_inv || !d->face_mass)
       return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv and face_mass must be given together");
     const std::string msg = build_adjoint_data(d, np, nfp, fm, nf, xfs);
     if (!msg.empty()) return fail(DG_ERR_INVALID_ARGUMENT, msg);
+    // A_r = D_r^T M, stored [r][j][i] = A_r[i][j]: the adjoint's volume
+    // operator acting on mu = M^{-1} lam
+    std::vector<double> mass;
+    if (!invert(np, d->mass_inv, mass)) return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv is singular");
+    const double* dop = d->equation == DG_EQ_MAXWELL ? d->d_strong : d->d_weak;
+    amat_t.assign((size_t)3 * np * np, 0.0);
+    for (int r = 0; r < 3; ++r)
+      for (int i = 0; i < np; ++i)
+        for (int j = 0; j < np; ++j) {
+          double acc = 0.0;
+          for (int kk = 0; kk < np; ++kk)
+            acc += dop[((size_t)r * np + kk) * np + i] * mass[(size_t)kk * np + j];
+          amat_t[((size_t)r * np + j) * np + i] = acc;
+        }
   }
 
   dg_plan* p = new (std::nothrow) dg_plan;
@@ -677,7 +743,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   p->nfields = nfields_of(d->equation);
   p->nelem = k;
   for (int t = 0; t < 4; ++t) p->params[t] = d->params[t];
-  p->variant = d->variant == DG_KERNEL_LDS ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
+  p->variant = d->variant == DG_KERNEL_AUTO ? auto_variant(d->equation, d->order) : d->variant;
   p->kernel = p->variant == DG_KERNEL_LDS ? pick_lds(d->equation, d->order)
                                           : pick_mfma(d->equation, d->order);
 
@@ -761,7 +827,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     p->mixed_grid[k] = (int)((mg + 7) / 8 * 8);
   }
   if (want_vjp) {
-    if ((rc = upload(&p->dmat, dsrc, (size_t)3 * np * np)) ||
+    if ((rc = upload(&p->amat_t, amat_t.data(), amat_t.size())) ||
         (rc = upload(&p->face_mass, fm.data(), fm.size())) ||
         (rc = upload(&p->mass_inv, d->mass_inv, (size_t)np * np)) ||
         (rc = upload(&p->ext_fscale, xfs.data(), xfs.size())) ||
@@ -769,20 +835,24 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       free_plan(p);
       return rc;
     }
-    p->vjp_kernel = pick_vjp(d->equation, d->order);
-    p->massinv_kernel = pick_mass_inv(d->order);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->vjp_kernel, kBlock, 0) !=
-            hipSuccess ||
-        occ < 1)
-      occ = 1;
+    p->vjp_kernel = pick_vjp<double>(d->equation, d->order);
+    p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order);
+    p->massinv_kernel = pick_mass_inv<double>(d->order);
+    p->massinv_kernel32 = pick_mass_inv<float>(d->order);
     const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
-    int64_t vg = (int64_t)prop.multiProcessorCount * occ;
-    if (vg > vbatch) vg = vbatch;
-    p->vjp_grid = (int)((vg + 7) / 8 * 8);
-    int64_t mg = (int64_t)prop.multiProcessorCount * 4;
-    if (mg > vbatch) mg = vbatch;
-    p->massinv_grid = (int)mg;
+    auto grid_of = [&](const void* fn) {
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kBlock, 0) != hipSuccess || occ < 1)
+        occ = 1;
+      int64_t g = (int64_t)prop.multiProcessorCount * occ;
+      if (g > vbatch) g = vbatch;
+      return (int)((g + 7) / 8 * 8);
+    };
+    p->vjp_grid = grid_of((const void*)p->vjp_kernel);
+    p->vjp_grid32 = grid_of((const void*)p->vjp_kernel32);
+    int64_t mg = (int64_t)prop.multiProcessorCount * 8;
+    const int64_t mneed = (k * np + kBlock - 1) / kBlock;
+    p->massinv_grid = (int)(mg < mneed ? mg : mneed);
     p->has_vjp = true;
   }
   *out = p;
@@ -800,6 +870,12 @@ int dg_plan_info(const dg_plan* p, int32_t* nfields, int32_t* np, int32_t* nfp, 
   if (np) *np = p->np;
   if (nfp) *nfp = p->nfp;
   if (nelem) *nelem = p->nelem;
+  return DG_OK;
+}
+
+int dg_plan_variant(const dg_plan* p, int32_t* variant) {
+  if (!p || !variant) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan or output");
+  *variant = p->variant;
   return DG_OK;
 }
 
@@ -859,11 +935,18 @@ int dg_lserk_stage(const dg_plan* p, double a, double b, double dt, double* cons
   return DG_OK;
 }
 
-int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rhs_bar,
-               double* const* q_bar, double* const* workspace, void* stream) {
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T* const* q_bar,
+                 T* const* workspace, void* stream, const char* name) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
   if (!p->has_vjp)
-    return fail(DG_ERR_UNSUPPORTED, "plan created without mass_inv / face_mass: no adjoint");
+    return fail(DG_ERR_UNSUPPORTED, std::string(name) +
+                                        ": plan created without mass_inv / face_mass: no adjoint");
   int rc;
   if (p->equation == DG_EQ_EULER && (rc = check_ptrs((const void* const*)q, p->nfields, "q")))
     return rc;
@@ -871,57 +954,95 @@ int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rh
       (rc = check_ptrs((const void* const*)q_bar, p->nfields, "q_bar")) ||
       (rc = check_ptrs((const void* const*)workspace, p->nfields, "workspace")))
     return rc;
-  VjpArgs a{};
-  OutPtrs mu{};
+  const bool f32 = sizeof(T) == 4;
+  MassInvArgs<T> m{};
+  VjpArgs<T> a{};
   for (int c = 0; c < p->nfields; ++c) {
-    a.q.p[c] = q ? q[c] : nullptr;
-    a.lam.p[c] = rhs_bar[c];
-    a.mu.p[c] = workspace[c];
-    a.qbar.p[c] = q_bar[c];
-    mu.p[c] = workspace[c];
+    m.lam[c] = rhs_bar[c];
+    m.mu[c] = workspace[c];
+    a.q[c] = q ? q[c] : nullptr;
+    a.mu[c] = workspace[c];
+    a.qbar[c] = q_bar[c];
   }
-  hipLaunchKernelGGL(p->massinv_kernel, dim3(p->massinv_grid), dim3(kBlock), 0,
-                     (hipStream_t)stream, a.lam, mu, p->nfields, p->mass_inv, p->nelem);
+  m.mass_inv = p->mass_inv;
+  m.nelem = p->nelem;
+  m.nf = p->nfields;
+  MassInvKernel<T> mk;
+  VjpKernel<T> vk;
+  if constexpr (sizeof(T) == 4) {
+    mk = p->massinv_kernel32;
+    vk = p->vjp_kernel32;
+  } else {
+    mk = p->massinv_kernel;
+    vk = p->vjp_kernel;
+  }
+  hipLaunchKernelGGL(mk, dim3(p->massinv_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
   DG_HIP_CHECK(hipGetLastError());
   a.sched = {p->sched_batch, p->part_batch};
   a.geo = p->geo;
   a.emap = p->emap;
   a.fmask = p->fmask;
-  a.dmat = p->dmat;
+  a.amat_t = p->amat_t;
   a.face_mass = p->face_mass;
-  a.mass_inv = p->mass_inv;
   a.ext_fscale = p->ext_fscale;
   a.node_faces = p->node_faces;
   a.nelem = p->nelem;
   a.p0 = p->params[0];
-  hipLaunchKernelGGL(p->vjp_kernel, dim3(p->vjp_grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(vk, dim3(f32 ? p->vjp_grid32 : p->vjp_grid), dim3(kBlock), 0,
+                     (hipStream_t)stream, a);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
 }
 
-int dg_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt, double* const* qbar,
-                           double* const* sv, const double* const* w, void* stream) {
+template <typename T>
+int lserk_adjoint_impl(const dg_plan* p, double a, double b, double dt, T* const* qbar,
+                       T* const* sv, const T* const* w, void* stream) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
   int rc;
   if ((rc = check_ptrs((const void* const*)qbar, p->nfields, "q_bar")) ||
       (rc = check_ptrs((const void* const*)sv, p->nfields, "s")) ||
       (w && (rc = check_ptrs((const void* const*)w, p->nfields, "w"))))
     return rc;
-  LserkAdjArgs s{};
+  LserkAdjArgs<T> s{};
   for (int c = 0; c < p->nfields; ++c) {
-    s.qbar.p[c] = qbar[c];
-    s.s.p[c] = sv[c];
-    s.w.p[c] = w ? w[c] : nullptr;
+    s.qbar[c] = qbar[c];
+    s.s[c] = sv[c];
+    s.w[c] = w ? w[c] : nullptr;
   }
   s.n = p->nelem * p->np;
   s.nf = p->nfields;
   s.a = a;
   s.b = b;
   s.dt = dt;
-  hipLaunchKernelGGL(lserk_adjoint_kernel, dim3(p->lserk_grid), dim3(kBlock), 0,
+  hipLaunchKernelGGL(lserk_adjoint_kernel<T>, dim3(p->lserk_grid), dim3(kBlock), 0,
                      (hipStream_t)stream, s);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rhs_bar,
+               double* const* q_bar, double* const* workspace, void* stream) {
+  return rhs_vjp_impl<double>(p, q, rhs_bar, q_bar, workspace, stream, "dg_rhs_vjp");
+}
+
+int dg_rhs_vjp_fp32(const dg_plan* p, const float* const* q, const float* const* rhs_bar,
+                    float* const* q_bar, float* const* workspace, void* stream) {
+  return rhs_vjp_impl<float>(p, q, rhs_bar, q_bar, workspace, stream, "dg_rhs_vjp_fp32");
+}
+
+int dg_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt, double* const* qbar,
+                           double* const* sv, const double* const* w, void* stream) {
+  return lserk_adjoint_impl<double>(p, a, b, dt, qbar, sv, w, stream);
+}
+
+int dg_lserk_stage_adjoint_fp32(const dg_plan* p, double a, double b, double dt,
+                                float* const* qbar, float* const* sv, const float* const* w,
+                                void* stream) {
+  return lserk_adjoint_impl<float>(p, a, b, dt, qbar, sv, w, stream);
 }
 
 }  // extern "C"

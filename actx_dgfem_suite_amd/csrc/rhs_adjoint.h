@@ -4,47 +4,62 @@
 // (linear), and of the Euler operator linearised at q (flux Jacobians
 // recomputed per node).
 //
-// Surface transpose without atomics.  The forward surface term is
-//   rhs_c[e,i] += sum_fn LIFT[i][fn] fs_e(f) Phi_c(qM, qP, n)
-// with LIFT = M^{-1} E and E the face mass.  Its transpose needs, per face
-// node, psi_e(fn) = fs_e(f) (E^T M^{-1} lam_e)(fn); E couples only the nodes of
-// one face, so with mu = M^{-1} lam precomputed per element (first kernel),
-//   psi_e(f, a) = fs_e(f) sum_b Mf_f[b][a] mu_e[fmask_f[b]].
-// The exterior map is an involution: the contribution of a neighbour's flux
-// to this element's face node is a GATHER of the neighbour's psi at the
-// matching face node, which in this element's face numbering is
+// Two passes.  The first forms mu = M^{-1} lam per element.  Everything the
+// second pass needs is a function of mu:
+//   volume:  D_r^T lam = (D_r^T M) mu = A_r mu       (A_r = S_r in weak form)
+//   surface: the forward term rhs_c[e,i] += sum_fn LIFT[i][fn] fs_e(f) Phi_c
+//            with LIFT = M^{-1} E transposes to the face cotangent
+//            psi_e(f, a) = fs_e(f) sum_b Mf_f[b][a] mu_e[fmask_f[b]]
+//            (E couples only the nodes of one face).
+// Surface transpose without atomics: the exterior map is an involution, so the
+// contribution of the neighbour's flux to this element's face node is a
+// GATHER of the neighbour's psi at the matching face node, which in this
+// element's face numbering is
 //   psi_e'(f, a) = fs_e'(f') sum_b Mf_f[b][a] mu_e'(ext(e, f, b))
 // (the face mass of one physical face is the same in both numberings), and
 //   q_bar(fmask[fn]) += JM^T psi_e + [boundary: B^T JP^T psi_e]
 //                     + JP(q_e'(ext), q_e(fn), -n)^T psi_e'.
-// Each element's face nodes are then summed into its volume nodes
-// (vertex nodes lie on three faces, edge nodes on two).
+// For the conservative wave and Euler fluxes Phi(a, b, -n) = -Phi(b, a, n),
+// so the last term is -JM^T psi_e' and one flux VJP with psi_e - psi_e'
+// covers both.  Each element's face nodes are then summed into its volume
+// nodes (vertex nodes lie on three faces, edge nodes on two).
 //
 // Euler's LF wavespeed max(lam_M, lam_P) takes the subgradient that splits
 // exact ties 0.5 / 0.5 (the convention of JAX's maximum); ties occur at wall
-// faces, where the reflected state has the same |u| and c.
+// faces, where both branches have the same derivative.
+//
+// State, cotangents and results are float64 or float32 (the FP32 adjoint);
+// the arithmetic is FP64 in both.
 #pragma once
 
 #include "dg_common.h"
 
 namespace dg {
 
+template <typename T>
 struct VjpArgs {
-  Schedule sched;    // batches of kBlock / Np elements
-  InPtrs q;          // state (Euler only; may be NULL for wave / Maxwell)
-  InPtrs lam;        // cotangent of the RHS
-  InPtrs mu;         // M^{-1} lam per element (workspace)
-  OutPtrs qbar;      // result
-  const double* geo;       // (K, 25)
-  const int32_t* emap;     // (K, 4*Nfp)
-  const int32_t* fmask;    // (4*Nfp)
-  const double* dmat;      // (3, Np, Np) D[r][j][i] row-major (weak, or strong for Maxwell)
-  const double* face_mass; // (4, Nfp, Nfp) Mf[f][b][a]
-  const double* mass_inv;  // (Np, Np) reference inverse mass
-  const double* ext_fscale;// (K, 4) Fscale of the neighbour across each face
-  const int32_t* node_faces;  // (Np, 3) face-node indices of each volume node, -1 padded
+  Schedule sched;                // batches of kBlock / Np elements
+  const T* q[kMaxFields];        // state (Euler only; NULL for wave / Maxwell)
+  const T* mu[kMaxFields];       // M^{-1} lam per element (first pass)
+  T* qbar[kMaxFields];           // result
+  const double* geo;             // (K, 25)
+  const int32_t* emap;           // (K, 4*Nfp)
+  const int32_t* fmask;          // (4*Nfp)
+  const double* amat_t;          // (3, Np(j), Np(i)): A_r[i][j] = (D_r^T M)[i][j]
+  const double* face_mass;       // (4, Nfp, Nfp) Mf[f][b][a]
+  const double* ext_fscale;      // (K, 4) Fscale of the neighbour across each face
+  const int32_t* node_faces;     // (Np, 3) face-node indices of each volume node, -1 padded
   int64_t nelem;
   double p0;
+};
+
+template <typename T>
+struct MassInvArgs {
+  const T* lam[kMaxFields];
+  T* mu[kMaxFields];
+  const double* mass_inv;        // (Np, Np), symmetric
+  int64_t nelem;
+  int nf;
 };
 
 // {{{ flux VJPs: given psi (cotangent of the flux 'out' of FluxOf<EQ>::flux),
@@ -53,6 +68,14 @@ struct VjpArgs {
 
 struct WaveAdj {
   static constexpr int NF = 4;
+  static constexpr bool CONSERVATIVE = true;
+  __device__ static void vjp_m(const double*, const double*, const double* n, double c,
+                               const double* psi, double* gM) {
+    const double pvn = psi[1] * n[0] + psi[2] * n[1] + psi[3] * n[2];
+    gM[0] = 0.5 * c * (-psi[0] + pvn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gM[1 + k] = 0.5 * c * (psi[0] - pvn) * n[k];
+  }
   __device__ static void vjp(const double*, const double*, const double* n, double c,
                              const double* psi, double* gM, double* gP) {
     const double pvn = psi[1] * n[0] + psi[2] * n[1] + psi[3] * n[2];
@@ -69,6 +92,9 @@ struct WaveAdj {
 
 struct MaxwellAdj {
   static constexpr int NF = 6;
+  static constexpr bool CONSERVATIVE = false;   // strong form: flux minus own trace
+  __device__ static void vjp_m(const double*, const double*, const double*, double,
+                               const double*, double*) {}
   __device__ static void cross(const double* a, const double* b, double* o) {
     o[0] = a[1] * b[2] - a[2] * b[1];
     o[1] = a[2] * b[0] - a[0] * b[2];
@@ -100,6 +126,7 @@ struct MaxwellAdj {
 
 struct EulerAdj {
   static constexpr int NF = 5;
+  static constexpr bool CONSERVATIVE = true;
   // g += (d (F(q).n) / d q)^T psi, n need not be a unit vector
   __device__ static void nflux_vjp(const double* q, const double* n, double gamma,
                                    const double* psi, double* g) {
@@ -137,8 +164,10 @@ struct EulerAdj {
       ds[2 + j] = (au > 0.0 ? u[j] / (au * rho) : 0.0) - k * (gamma - 1.0) * u[j];
     return au + c;
   }
-  __device__ static void vjp(const double* qM, const double* qP, const double* n, double gamma,
-                             const double* psi, double* gM, double* gP) {
+  // gM (and gP when NEED_P) of the LF flux
+  template <bool NEED_P>
+  __device__ static void vjp_impl(const double* qM, const double* qP, const double* n,
+                                  double gamma, const double* psi, double* gM, double* gP) {
     double dsM[5], dsP[5];
     const double lM = speed(qM, gamma, dsM);
     const double lP = speed(qP, gamma, dsP);
@@ -153,10 +182,18 @@ struct EulerAdj {
     for (int c = 0; c < 5; ++c) {
       hpsi[c] = -0.5 * psi[c];
       gM[c] = -0.5 * lam * psi[c] + 0.5 * jump * wM * dsM[c];
-      gP[c] = 0.5 * lam * psi[c] + 0.5 * jump * wP * dsP[c];
+      if constexpr (NEED_P) gP[c] = 0.5 * lam * psi[c] + 0.5 * jump * wP * dsP[c];
     }
     nflux_vjp(qM, n, gamma, hpsi, gM);
-    nflux_vjp(qP, n, gamma, hpsi, gP);
+    if constexpr (NEED_P) nflux_vjp(qP, n, gamma, hpsi, gP);
+  }
+  __device__ static void vjp(const double* qM, const double* qP, const double* n, double gamma,
+                             const double* psi, double* gM, double* gP) {
+    vjp_impl<true>(qM, qP, n, gamma, psi, gM, gP);
+  }
+  __device__ static void vjp_m(const double* qM, const double* qP, const double* n, double gamma,
+                               const double* psi, double* gM) {
+    vjp_impl<false>(qM, qP, n, gamma, psi, gM, nullptr);
   }
   __device__ static void bcT(const double* n, double* g) {
     const double gn = g[2] * n[0] + g[3] * n[1] + g[4] * n[2];
@@ -173,32 +210,28 @@ template <> struct AdjOf<DG_EQ_MAXWELL> { using type = MaxwellAdj; };
 
 // }}}
 
-// {{{ mu = M^{-1} lam per element (reference inverse mass)
+// {{{ mu = M^{-1} lam per element: one thread per (element, node), the
+// element's lam row read straight from memory (the threads of one element
+// share its cache lines), M^{-1} in LDS column-major so lanes read
+// consecutive words
 
-template <int ORDER>
-__global__ __launch_bounds__(kBlock) void mass_inv_kernel(InPtrs lam, OutPtrs mu, int nf,
-                                                          const double* mass_inv, int64_t nelem) {
+template <int ORDER, typename T>
+__global__ __launch_bounds__(kBlock) void mass_inv_kernel(MassInvArgs<T> a) {
   constexpr int NP = np_of(ORDER);
-  constexpr int EB = kBlock / NP;
-  __shared__ double s_m[NP * NP];
-  __shared__ double s_l[EB * NP];
-  const int tid = threadIdx.x;
-  for (int k = tid; k < NP * NP; k += kBlock) s_m[k] = mass_inv[k];
-  const int el = tid / NP, i = tid - el * NP;
-  const int64_t nbatch = (nelem + EB - 1) / EB;
-  for (int c = 0; c < nf; ++c) {
-    for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
-      const int64_t e0 = b * EB;
-      const int nel = (int)(nelem - e0 < EB ? nelem - e0 : EB);
-      __syncthreads();
-      for (int k = tid; k < nel * NP; k += kBlock) s_l[k] = lam.p[c][e0 * NP + k];
-      __syncthreads();
-      if (el < nel) {
-        double acc = 0.0;
-#pragma unroll 5
-        for (int j = 0; j < NP; ++j) acc += s_m[i * NP + j] * s_l[el * NP + j];
-        mu.p[c][e0 * NP + tid] = acc;
-      }
+  __shared__ double s_m[NP * NP];     // s_m[j * NP + i] = Minv[i][j] (= Minv[j][i])
+  for (int k = threadIdx.x; k < NP * NP; k += kBlock) s_m[k] = a.mass_inv[k];
+  __syncthreads();
+  const int64_t total = a.nelem * NP;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < total; idx += stride) {
+    const int64_t e = idx / NP;
+    const int i = (int)(idx - e * NP);
+    for (int c = 0; c < a.nf; ++c) {
+      const T* row = a.lam[c] + e * NP;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) acc += s_m[j * NP + i] * (double)row[j];
+      a.mu[c][idx] = (T)acc;
     }
   }
 }
@@ -207,8 +240,8 @@ __global__ __launch_bounds__(kBlock) void mass_inv_kernel(InPtrs lam, OutPtrs mu
 
 // {{{ adjoint kernel: one thread per (element, node), element batches in LDS
 
-template <int EQ, int ORDER>
-__global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
+template <int EQ, int ORDER, typename T>
+__global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
   using Adj = typename AdjOf<EQ>::type;
   using Flux = typename FluxOf<EQ>::type;
   constexpr int NP = np_of(ORDER);
@@ -220,22 +253,22 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
   constexpr int NQ = NEEDS_Q ? NF : 1;
   constexpr int NW = EQ == DG_EQ_WAVE ? 3 : 1;
 
-  __shared__ double s_d[3 * NP * NP];        // D[r][j][i]
+  __shared__ double s_a[3 * NP * NP];        // A_r[i][j] at [r][j][i]
   __shared__ double s_mf[kFaces * NFP * NFP];
   __shared__ int s_fmask[NFN];
   __shared__ int s_nf[NP * 3];
-  __shared__ double s_lam[NF][EB * NP];
   __shared__ double s_mu[NF][EB * NP];
   __shared__ double s_q[NQ][EB * NP];
   __shared__ double s_w[NW][EB * NP];
   __shared__ double s_mux[NF][EB * NFN];     // mu at exterior nodes
-  __shared__ double s_qx[NQ][EB * NFN];      // q at exterior nodes (Euler)
-  __shared__ double s_g[NF][EB * NFN];       // face-node contributions to q_bar
+  // q at exterior nodes (Euler), overwritten in place (same thread, same
+  // slot) by the face-node contributions to q_bar
+  __shared__ double s_x[NF][EB * NFN];
   __shared__ double s_geo[EB * kGeo];
   __shared__ double s_xfs[EB * kFaces];
 
   const int tid = threadIdx.x;
-  for (int k = tid; k < 3 * NP * NP; k += kBlock) s_d[k] = a.dmat[k];
+  for (int k = tid; k < 3 * NP * NP; k += kBlock) s_a[k] = a.amat_t[k];
   for (int k = tid; k < kFaces * NFP * NFP; k += kBlock) s_mf[k] = a.face_mass[k];
   for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
   for (int k = tid; k < NP * 3; k += kBlock) s_nf[k] = a.node_faces[k];
@@ -254,9 +287,8 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
 #pragma unroll
     for (int c = 0; c < NF; ++c)
       for (int k = tid; k < nel * NP; k += kBlock) {
-        s_lam[c][k] = a.lam.p[c][e0 * NP + k];
-        s_mu[c][k] = a.mu.p[c][e0 * NP + k];
-        if constexpr (NEEDS_Q) s_q[c][k] = a.q.p[c][e0 * NP + k];
+        s_mu[c][k] = (double)a.mu[c][e0 * NP + k];
+        if constexpr (NEEDS_Q) s_q[c][k] = (double)a.q[c][e0 * NP + k];
       }
     for (int k = tid; k < nel * kGeo; k += kBlock) s_geo[k] = a.geo[e0 * kGeo + k];
     for (int k = tid; k < nel * kFaces; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * kFaces + k];
@@ -265,8 +297,8 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
       const int32_t ext = a.emap[e0 * NFN + k];
 #pragma unroll
       for (int c = 0; c < NF; ++c) {
-        s_mux[c][k] = ext >= 0 ? a.mu.p[c][ext] : 0.0;
-        if constexpr (NEEDS_Q) s_qx[c][k] = ext >= 0 ? a.q.p[c][ext] : 0.0;
+        s_mux[c][k] = ext >= 0 ? (double)a.mu[c][ext] : 0.0;
+        if constexpr (NEEDS_Q) s_x[c][k] = ext >= 0 ? (double)a.q[c][ext] : 0.0;
       }
     }
     __syncthreads();
@@ -278,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
         const int node = el * NP + i;
 #pragma unroll
         for (int r = 0; r < 3; ++r)
-          s_w[r][node] = g[r] * s_lam[1][node] + g[3 + r] * s_lam[2][node] + g[6 + r] * s_lam[3][node];
+          s_w[r][node] = g[r] * s_mu[1][node] + g[3 + r] * s_mu[2][node] + g[6 + r] * s_mu[3][node];
       }
     }
 
@@ -290,7 +322,6 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
       const int aa = fn - f * NFP;
       const double* geo = &s_geo[fe * kGeo + 9 + 4 * f];
       const double n[3] = {geo[0], geo[1], geo[2]};
-      const double nm[3] = {-n[0], -n[1], -n[2]};
       const double fs = geo[3];
       const double xfs = s_xfs[fe * kFaces + f];
       double psi[NF], psin[NF];
@@ -310,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
 #pragma unroll
       for (int c = 0; c < NF; ++c) {
         qm[c] = NEEDS_Q ? s_q[c < NQ ? c : 0][fe * NP + s_fmask[fn]] : 0.0;
-        qp[c] = NEEDS_Q ? s_qx[c < NQ ? c : 0][k] : 0.0;
+        qp[c] = NEEDS_Q ? s_x[c][k] : 0.0;
       }
       if (ext < 0) {
         if constexpr (NEEDS_Q) Flux::boundary(qm, n, qp);
@@ -318,7 +349,13 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
         Adj::bcT(n, gP);
 #pragma unroll
         for (int c = 0; c < NF; ++c) gM[c] += gP[c];
+      } else if constexpr (Adj::CONSERVATIVE) {
+        double pd[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) pd[c] = psi[c] - psin[c];
+        Adj::vjp_m(qm, qp, n, c0, pd, gM);
       } else {
+        const double nm[3] = {-n[0], -n[1], -n[2]};
         Adj::vjp(qm, qp, n, c0, psi, gM, gP);
         double hM[NF], hP[NF];
         // the neighbour's flux: its interior is qp, its exterior is our qm
@@ -327,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
         for (int c = 0; c < NF; ++c) gM[c] += hP[c];
       }
 #pragma unroll
-      for (int c = 0; c < NF; ++c) s_g[c][k] = gM[c];
+      for (int c = 0; c < NF; ++c) s_x[c][k] = gM[c];
     }
     __syncthreads();
 
@@ -336,33 +373,33 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
       const int base = el * NP;
       double acc[NF];
       if constexpr (EQ == DG_EQ_WAVE) {
-        // u_bar = -c sum_r D_r^T (sum_x rx[x][r] lam_vx); v_x_bar = -c sum_r rx[x][r] D_r^T lam_u
+        // u_bar = -c sum_r A_r (sum_x rx[x][r] mu_vx); v_x_bar = -c sum_r rx[x][r] A_r mu_u
         double au = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_d[j * NP + i];
-          const double d1 = s_d[NP * NP + j * NP + i];
-          const double d2 = s_d[2 * NP * NP + j * NP + i];
-          const double lu = s_lam[0][base + j];
+          const double d0 = s_a[j * NP + i];
+          const double d1 = s_a[NP * NP + j * NP + i];
+          const double d2 = s_a[2 * NP * NP + j * NP + i];
+          const double mu = s_mu[0][base + j];
           au += d0 * s_w[0][base + j] + d1 * s_w[1][base + j] + d2 * s_w[2][base + j];
-          t0 += d0 * lu;
-          t1 += d1 * lu;
-          t2 += d2 * lu;
+          t0 += d0 * mu;
+          t1 += d1 * mu;
+          t2 += d2 * mu;
         }
         acc[0] = -c0 * au;
 #pragma unroll
         for (int x = 0; x < 3; ++x) acc[1 + x] = -c0 * (g[3 * x] * t0 + g[3 * x + 1] * t1 + g[3 * x + 2] * t2);
       } else if constexpr (EQ == DG_EQ_EULER) {
-        // T_{c,r} = D_r^T lam_c; Fbar_{c,x} = sum_r rx[x][r] T_{c,r}; q_bar = sum_x (dF_x/dq)^T Fbar_x
+        // T_{c,r} = A_r mu_c; Fbar_{c,x} = sum_r rx[x][r] T_{c,r}; q_bar = sum_x (dF_x/dq)^T Fbar_x
         double t[5][3];
 #pragma unroll
         for (int c = 0; c < 5; ++c) t[c][0] = t[c][1] = t[c][2] = 0.0;
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_d[j * NP + i];
-          const double d1 = s_d[NP * NP + j * NP + i];
-          const double d2 = s_d[2 * NP * NP + j * NP + i];
+          const double d0 = s_a[j * NP + i];
+          const double d1 = s_a[NP * NP + j * NP + i];
+          const double d2 = s_a[2 * NP * NP + j * NP + i];
 #pragma unroll
           for (int c = 0; c < 5; ++c) {
-            const double l = s_lam[c][base + j];
+            const double l = s_mu[c][base + j];
             t[c][0] += d0 * l;
             t[c][1] += d1 * l;
             t[c][2] += d2 * l;
@@ -388,12 +425,12 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) t[c][0] = t[c][1] = t[c][2] = 0.0;
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_d[j * NP + i];
-          const double d1 = s_d[NP * NP + j * NP + i];
-          const double d2 = s_d[2 * NP * NP + j * NP + i];
+          const double d0 = s_a[j * NP + i];
+          const double d1 = s_a[NP * NP + j * NP + i];
+          const double d2 = s_a[2 * NP * NP + j * NP + i];
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
-            const double l = s_lam[c][base + j];
+            const double l = s_mu[c][base + j];
             t[c][0] += d0 * l;
             t[c][1] += d1 * l;
             t[c][2] += d2 * l;
@@ -424,12 +461,12 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs a) {
         const int fn = s_nf[i * 3 + s];
         if (fn >= 0) {
 #pragma unroll
-          for (int c = 0; c < NF; ++c) acc[c] += s_g[c][el * NFN + fn];
+          for (int c = 0; c < NF; ++c) acc[c] += s_x[c][el * NFN + fn];
         }
       }
       const int64_t out = e0 * NP + tid;
 #pragma unroll
-      for (int c = 0; c < NF; ++c) a.qbar.p[c][out] = acc[c];
+      for (int c = 0; c < NF; ++c) a.qbar[c][out] = (T)acc[c];
     }
   }
 }

@@ -193,10 +193,14 @@ def get_roofline_flop_rate(equation: str, dim: int, degree: int,
     )
     if roofline_model != "libparanumal:global_ai":
         raise NotImplementedError(f"unknown roofline model: {roofline_model}")
+    arch = ""
     if device_name is None:
         import torch
-        device_name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else ""
-    key = lookup_device(device_name)
+        device_name = ""
+        if torch.cuda.is_available():
+            device_name = torch.cuda.get_device_name(0)
+            arch = getattr(torch.cuda.get_device_properties(0), "gcnArchName", "")
+    key = lookup_device(device_name, arch)
     if key is None:
         return float("nan")
     nflops = get_float64_flops(equation, dim, degree)

@@ -83,8 +83,16 @@ class DGPlan:
                                                          ctypes.byref(handle)),
                        "dg_plan_create")
         self._handle = handle
-        self._vjp_workspace: list[torch.Tensor] | None = None
+        self._vjp_workspace: dict[torch.dtype, list[torch.Tensor]] = {}
         del self._host  # the plan holds device copies
+
+    @property
+    def kernel_variant(self) -> str:
+        """The FP64 kernel this plan runs: ``"mfma"`` or ``"lds"``."""
+        v = ctypes.c_int32()
+        _lib.check(self.lib, self.lib.dg_plan_variant(self._handle, ctypes.byref(v)),
+                   "dg_plan_variant")
+        return {_lib.DG_KERNEL_MFMA: "mfma", _lib.DG_KERNEL_LDS: "lds"}[v.value]
 
     # {{{ validation
 
@@ -185,46 +193,62 @@ class DGPlan:
 
     def rhs_vjp(self, q: Sequence[torch.Tensor] | None, rhs_bar: Sequence[torch.Tensor],
                 out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:
-        """Vector-Jacobian product ``q_bar = (dL/dq)^T rhs_bar`` at state ``q``
-        (FP64).  ``q`` is read by Euler only (wave and Maxwell are linear) and
-        may be ``None`` for them.  Launches on the current stream; the
-        workspace is allocated on the first call and reused."""
+        """Vector-Jacobian product ``q_bar = (dL/dq)^T rhs_bar`` at state ``q``.
+
+        float64 fields use ``dg_rhs_vjp``; float32 fields the FP32 adjoint
+        ``dg_rhs_vjp_fp32`` (FP64 arithmetic, FP32 storage).  ``q`` is read by
+        Euler only (wave and Maxwell are linear) and may be ``None`` for them.
+        Launches on the current stream; the workspace is allocated on the
+        first call per dtype and reused."""
+        dtype = rhs_bar[0].dtype if len(rhs_bar) else torch.float64
+        if dtype not in (torch.float64, torch.float32):
+            raise TypeError(f"rhs_vjp: float64 or float32 fields, got {dtype}")
         if q is not None:
-            self._check_fields(q, "q")
+            self._check_fields(q, "q", dtype)
         elif self.equation == "euler":
             raise ValueError("rhs_vjp: the Euler adjoint needs the state q")
-        self._check_fields(rhs_bar, "rhs_bar")
+        self._check_fields(rhs_bar, "rhs_bar", dtype)
         if out is None:
-            out = self.empty_fields()
+            out = self.empty_fields(dtype)
         else:
-            self._check_fields(out, "out")
-        if self._vjp_workspace is None:
-            self._vjp_workspace = self.empty_fields()
+            self._check_fields(out, "out", dtype)
+        ws = self._vjp_workspace.get(dtype)
+        if ws is None:
+            ws = self._vjp_workspace[dtype] = self.empty_fields(dtype)
+        fn, name = ((self.lib.dg_rhs_vjp, "dg_rhs_vjp") if dtype == torch.float64
+                    else (self.lib.dg_rhs_vjp_fp32, "dg_rhs_vjp_fp32"))
         stream = torch.cuda.current_stream(self.device).cuda_stream
         pq = None if q is None else _lib.pointer_array([t.data_ptr() for t in q])
-        _lib.check(self.lib, self.lib.dg_rhs_vjp(
+        _lib.check(self.lib, fn(
             self._handle, pq,
             _lib.pointer_array([t.data_ptr() for t in rhs_bar]),
             _lib.pointer_array([t.data_ptr() for t in out]),
-            _lib.pointer_array([t.data_ptr() for t in self._vjp_workspace]),
-            ctypes.c_void_p(stream)), "dg_rhs_vjp")
+            _lib.pointer_array([t.data_ptr() for t in ws]),
+            ctypes.c_void_p(stream)), name)
         return list(out)
 
     def lserk_stage_adjoint(self, a: float, b: float, dt: float, q_bar: Sequence[torch.Tensor],
                             s: Sequence[torch.Tensor],
                             w: Sequence[torch.Tensor] | None) -> None:
-        """Reverse sweep of one LSERK4 stage: ``q_bar += dt*w; s = a*s + b*q_bar``."""
-        self._check_fields(q_bar, "q_bar")
-        self._check_fields(s, "s")
+        """Reverse sweep of one LSERK4 stage: ``q_bar += dt*w; s = a*s + b*q_bar``
+        (float64, or float32 for the FP32 adjoint)."""
+        dtype = q_bar[0].dtype if len(q_bar) else torch.float64
+        if dtype not in (torch.float64, torch.float32):
+            raise TypeError(f"lserk_stage_adjoint: float64 or float32 fields, got {dtype}")
+        self._check_fields(q_bar, "q_bar", dtype)
+        self._check_fields(s, "s", dtype)
         if w is not None:
-            self._check_fields(w, "w")
+            self._check_fields(w, "w", dtype)
+        fn, name = ((self.lib.dg_lserk_stage_adjoint, "dg_lserk_stage_adjoint")
+                    if dtype == torch.float64
+                    else (self.lib.dg_lserk_stage_adjoint_fp32, "dg_lserk_stage_adjoint_fp32"))
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        _lib.check(self.lib, self.lib.dg_lserk_stage_adjoint(
+        _lib.check(self.lib, fn(
             self._handle, float(a), float(b), float(dt),
             _lib.pointer_array([t.data_ptr() for t in q_bar]),
             _lib.pointer_array([t.data_ptr() for t in s]),
             None if w is None else _lib.pointer_array([t.data_ptr() for t in w]),
-            ctypes.c_void_p(stream)), "dg_lserk_stage_adjoint")
+            ctypes.c_void_p(stream)), name)
 
     def close(self) -> None:
         h = getattr(self, "_handle", None)

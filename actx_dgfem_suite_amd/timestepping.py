@@ -57,62 +57,104 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
 
 
 class LSERK4:
-    """Forward and reverse LSERK4 steps of one plan, FP64, with buffers
-    allocated once."""
+    """Forward and reverse LSERK4 steps of one plan, with buffers allocated
+    once.
 
-    def __init__(self, plan, dt: float):
+    ``precision`` is the forward RHS: ``"fp64"``, or the mixed ``"fp32"`` /
+    ``"bf16x3"`` forms, which keep the state and residual in FP64 and feed the
+    RHS an FP32 shadow of the state (``dg_lserk_stage_mixed``).  The reverse
+    step runs in the dtype of the cotangent it is given: float64
+    (``dg_rhs_vjp``) or float32 (the FP32 adjoint, ``dg_rhs_vjp_fp32``).
+    """
+
+    def __init__(self, plan, dt: float, precision: str = "fp64"):
+        if precision not in ("fp64", "fp32", "bf16x3"):
+            raise ValueError(f"precision '{precision}'")
         self.plan = plan
         self.dt = float(dt)
+        self.precision = precision
         self._res = plan.empty_fields()
-        self._rhs = plan.empty_fields()
-        self._s = plan.empty_fields()
-        self._w = plan.empty_fields()
+        if precision == "fp64":
+            self._rhs = plan.empty_fields()
+            self._q32 = None
+        else:
+            self._rhs = plan.empty_fields(torch.float32)
+            self._q32 = plan.empty_fields(torch.float32)
+        self._adj: dict[torch.dtype, tuple[list[torch.Tensor], list[torch.Tensor]]] = {}
+
+    @property
+    def linear(self) -> bool:
+        """The operator is linear: its VJP does not read the state."""
+        return self.plan.equation != "euler"
 
     def step(self, q: Sequence[torch.Tensor],
              stages: list[list[torch.Tensor]] | None = None) -> None:
-        """Advance ``q`` in place by one step.  With ``stages`` given, the
-        five stage inputs are appended to it (copies)."""
+        """Advance the FP64 state ``q`` in place by one step.  With ``stages``
+        given, the five stage inputs are appended to it (copies)."""
         plan = self.plan
+        if self._q32 is None:
+            for a, b in zip(LSERK4_A, LSERK4_B):
+                if stages is not None:
+                    stages.append(_clone(q))
+                plan.rhs(q, self._rhs)
+                plan.lserk_stage(a, b, self.dt, q, self._res, self._rhs)
+            return
+        for t, src in zip(self._q32, q):
+            t.copy_(src)
         for a, b in zip(LSERK4_A, LSERK4_B):
             if stages is not None:
                 stages.append(_clone(q))
-            plan.rhs(q, self._rhs)
-            plan.lserk_stage(a, b, self.dt, q, self._res, self._rhs)
+            plan.rhs_mixed(self._q32, self._rhs, self.precision)
+            plan.lserk_stage_mixed(a, b, self.dt, q, self._res, self._rhs, self._q32)
 
     def run(self, q: Sequence[torch.Tensor], nsteps: int) -> None:
         for _ in range(nsteps):
             self.step(q)
 
-    def step_vjp(self, q_n: Sequence[torch.Tensor], q_bar: Sequence[torch.Tensor]) -> None:
+    def _adjoint_buffers(self, dtype):
+        buf = self._adj.get(dtype)
+        if buf is None:
+            buf = self._adj[dtype] = (self.plan.empty_fields(dtype), self.plan.empty_fields(dtype))
+        return buf
+
+    def step_vjp(self, q_n: Sequence[torch.Tensor] | None, q_bar: Sequence[torch.Tensor]) -> None:
         """Replace ``q_bar`` (the cotangent of q^{n+1}) in place by that of
-        q^n; ``q_n`` is the state at the start of the step (left intact)."""
+        q^n.  ``q_n`` is the FP64 state at the start of the step (left
+        intact); linear operators do not read it and accept ``None``."""
         plan = self.plan
-        nonlinear = plan.equation == "euler"
+        dtype = q_bar[0].dtype
+        s, w = self._adjoint_buffers(dtype)
         stages: list[list[torch.Tensor]] | None = None
-        if nonlinear:
+        if not self.linear:
+            if q_n is None:
+                raise ValueError("step_vjp: the Euler adjoint needs the step's start state")
             stages = []
             self.step(_clone(q_n), stages)
-        plan.lserk_stage_adjoint(0.0, LSERK4_B[4], self.dt, q_bar, self._s, None)
+            if dtype != torch.float64:
+                stages = [[t.to(dtype) for t in st] for st in stages]
+        plan.lserk_stage_adjoint(0.0, LSERK4_B[4], self.dt, q_bar, s, None)
         for k in range(4, -1, -1):
-            plan.rhs_vjp(stages[k] if nonlinear else None, self._s, self._w)
+            plan.rhs_vjp(None if stages is None else stages[k], s, w)
             a = LSERK4_A[k]
             b = LSERK4_B[k - 1] if k > 0 else 0.0
-            plan.lserk_stage_adjoint(a, b, self.dt, q_bar, self._s, self._w)
+            plan.lserk_stage_adjoint(a, b, self.dt, q_bar, s, w)
 
 
 def forward_with_checkpoints(stepper: LSERK4, q0: Sequence[torch.Tensor], nsteps: int,
                              checkpoint_every: int = 10
                              ) -> tuple[list[torch.Tensor], dict[int, list[torch.Tensor]]]:
     """Run ``nsteps`` steps from ``q0`` (left intact), keeping the state at
-    step 0 and every ``checkpoint_every`` steps.  Returns ``(q_final,
-    checkpoints)``."""
+    step 0 and every ``checkpoint_every`` steps -- for the nonlinear Euler
+    operator only: the linear operators' adjoint reads no forward state, so
+    they keep none.  Returns ``(q_final, checkpoints)``."""
     if checkpoint_every < 1:
         raise ValueError("checkpoint_every must be >= 1")
+    keep = not stepper.linear
     q = _clone(q0)
-    checkpoints = {0: _clone(q0)}
+    checkpoints = {0: _clone(q0)} if keep else {}
     for n in range(nsteps):
         stepper.step(q)
-        if (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
+        if keep and (n + 1) % checkpoint_every == 0 and n + 1 < nsteps:
             checkpoints[n + 1] = _clone(q)
     return q, checkpoints
 
@@ -120,11 +162,18 @@ def forward_with_checkpoints(stepper: LSERK4, q0: Sequence[torch.Tensor], nsteps
 def reverse_sweep(stepper: LSERK4, checkpoints: dict[int, list[torch.Tensor]], nsteps: int,
                   q_bar_final: Sequence[torch.Tensor],
                   checkpoint_every: int = 10) -> list[torch.Tensor]:
-    """Cotangent of the initial state from that of the final one.  Each
-    segment is recomputed from its checkpoint (its step-start states held on
-    the device) and its steps are transposed last to first.  Consumes
+    """Cotangent of the initial state from that of the final one (in the
+    final cotangent's dtype).  For the nonlinear Euler operator each segment
+    is recomputed from its checkpoint (its step-start states held on the
+    device) and its steps are transposed last to first; the linear operators'
+    adjoint reads no state, so their steps are transposed directly.  Consumes
     ``checkpoints``."""
     lam = _clone(q_bar_final)
+    if stepper.linear:
+        checkpoints.clear()
+        for _ in range(nsteps):
+            stepper.step_vjp(None, lam)
+        return lam
     for start in reversed(range(0, nsteps, checkpoint_every)):
         end = min(start + checkpoint_every, nsteps)
         states = [checkpoints.pop(start)]
@@ -145,3 +194,19 @@ def trajectory_vjp(stepper: LSERK4, q0: Sequence[torch.Tensor], nsteps: int,
     checkpoint_every`` states."""
     q_final, checkpoints = forward_with_checkpoints(stepper, q0, nsteps, checkpoint_every)
     return q_final, reverse_sweep(stepper, checkpoints, nsteps, q_bar_final, checkpoint_every)
+
+
+def misfit_gradient(stepper: LSERK4, q0: Sequence[torch.Tensor], target: Sequence[torch.Tensor],
+                    nsteps: int, checkpoint_every: int = 10,
+                    adjoint_dtype: torch.dtype = torch.float64
+                    ) -> tuple[float, list[torch.Tensor]]:
+    """``J = 1/2 sum_c ||q_c(T) - target_c||^2`` over the first
+    ``len(target)`` fields after ``nsteps`` steps from ``q0``, and its
+    gradient with respect to every field of ``q0`` (in ``adjoint_dtype``).
+    The forward runs in the stepper's precision."""
+    q_final, checkpoints = forward_with_checkpoints(stepper, q0, nsteps, checkpoint_every)
+    diff = [a - b for a, b in zip(q_final, target)]
+    j = 0.5 * float(sum(torch.sum(d * d) for d in diff))
+    q_bar = [(diff[c] if c < len(diff) else torch.zeros_like(q_final[c])).to(adjoint_dtype)
+             for c in range(len(q_final))]
+    return j, reverse_sweep(stepper, checkpoints, nsteps, q_bar, checkpoint_every)

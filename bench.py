@@ -230,7 +230,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": (f"rhs_mixed_kernel<{w.equation},{args.precision},P{w.order}>" if mixed else
-                       f"rhs_{'lds' if args.variant == 'lds' else 'mfma'}_kernel<{w.equation},P{w.order}>"),
+                       f"rhs_{plan.kernel_variant}_kernel<{w.equation},P{w.order}>"),
             "kernel_us": kernel_ms * 1e3,
             "algorithmic_bytes": b_alg,
             "algorithmic_flops": f_alg,

@@ -46,7 +46,7 @@ typedef enum dg_equation {
 } dg_equation;
 
 typedef enum dg_kernel_variant {
-  DG_KERNEL_AUTO = 0,   /* the fastest variant (currently MFMA) */
+  DG_KERNEL_AUTO = 0,   /* the measured-fastest per equation and order */
   DG_KERNEL_MFMA = 1,   /* fused kernel on v_mfma_f64_16x16x4f64 */
   DG_KERNEL_LDS = 2     /* one thread per node, LDS-staged (baseline) */
 } dg_kernel_variant;
@@ -111,6 +111,10 @@ int dg_plan_destroy(dg_plan* plan);
 int dg_plan_info(const dg_plan* plan, int32_t* nfields, int32_t* np,
                  int32_t* nfp, int64_t* nelements);
 
+/* The FP64 kernel variant the plan runs (DG_KERNEL_MFMA or DG_KERNEL_LDS;
+ * DG_KERNEL_AUTO resolves per equation and order). */
+int dg_plan_variant(const dg_plan* plan, int32_t* variant);
+
 /* One right-hand-side evaluation, rhs = L(q): the call `compiled_rhs(*args)`
  * (run/__main__.py:76,109,126) of an array context, i.e. wave_operator
  * (equations/wave.py:121-200), EulerOperator.operator (equations/euler.py:
@@ -161,6 +165,13 @@ int dg_rhs_vjp(const dg_plan* plan, const double* const* q,
                const double* const* rhs_bar, double* const* q_bar,
                double* const* workspace, void* stream);
 
+/* dg_rhs_vjp with float32 state, cotangent, result and workspace (the FP32
+ * adjoint); the arithmetic is FP64 as in dg_rhs_vjp.  No counterpart in the
+ * reference (see dg_rhs_vjp). */
+int dg_rhs_vjp_fp32(const dg_plan* plan, const float* const* q,
+                    const float* const* rhs_bar, float* const* q_bar,
+                    float* const* workspace, void* stream);
+
 /* Reverse sweep of one LSERK4 stage (the transpose of dg_lserk_stage):
  *   q_bar = q_bar + dt*w;   s = a*s + b*q_bar
  * where w = dg_rhs_vjp(q_k, s) of the stage.  w may be NULL (the dt*w term is
@@ -170,6 +181,12 @@ int dg_rhs_vjp(const dg_plan* plan, const double* const* q,
 int dg_lserk_stage_adjoint(const dg_plan* plan, double a, double b, double dt,
                            double* const* q_bar, double* const* s,
                            const double* const* w, void* stream);
+
+/* dg_lserk_stage_adjoint on float32 q_bar, s and w.  No counterpart in the
+ * reference (see dg_lserk_stage). */
+int dg_lserk_stage_adjoint_fp32(const dg_plan* plan, double a, double b, double dt,
+                                float* const* q_bar, float* const* s,
+                                const float* const* w, void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* dg_last_error_string(void);

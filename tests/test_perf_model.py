@@ -77,6 +77,8 @@ def test_roofline_rate_uses_the_peak_table():
     assert rate == pytest.approx(f / max(f / 78.6e12, b / 8.0e12))
     assert np.isnan(get_roofline_flop_rate("wave", 3, 1, device_name="some other device"))
     assert lookup_device("AMD Instinct MI355X OAM") == "AMD Instinct MI355X"
+    assert lookup_device("AMD Radeon Graphics", "gfx950:sramecc+:xnack-") == "AMD Instinct MI355X"
+    assert lookup_device("AMD Radeon Graphics", "gfx942") is None
 
 
 def test_unknown_names_raise():
