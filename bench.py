@@ -47,6 +47,23 @@ def _parse():
     return p.parse_args()
 
 
+def measured_traffic(workload: str, kernel: str) -> int | None:
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    copy of known size), or None when the summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fp64.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == workload and cells[1] == kernel:
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(workload, seconds: float):
     """Time the numpy oracle on a bounded sample: the same equation and order
     on a smaller box, reported as evaluations/s of the full workload (scaled
@@ -181,13 +198,16 @@ def main():
         del lam, qbar
 
     if args.copy_probe:
-        n = 256 * 1024 * 1024 // 8
+        # 1 GiB: larger than the 256 MB last-level cache, so the copy's reads
+        # come from HBM (the calibration of the FETCH_SIZE counter)
+        n = 1024 * 1024 * 1024 // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] copy probe: {n * 8} bytes read + written x3", file=sys.stderr)
+        del src, dst
 
     b_alg = algorithmic_bytes(k, npn, nfp, nf, state_bytes=4 if mixed else 8)
     f_alg = algorithmic_flops(w.equation, k, npn, nfp, nf)
@@ -196,6 +216,8 @@ def main():
     t_roof = max(b_alg / (HBM_PEAK_GBS * 1e9), f_alg / (FP64_PEAK_TFLOPS * 1e12))
     ms_per_step = wall / args.steps * 1e3
 
+    kernel_name = (f"rhs_mixed_kernel<{w.equation},{args.precision},P{w.order}>" if mixed else
+                   f"rhs_{plan.kernel_variant}_kernel<{w.equation},P{w.order}>")
     result = {
         "metric": "DG RHS evals/s",
         "value": args.steps / wall,
@@ -228,9 +250,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": (f"rhs_mixed_kernel<{w.equation},{args.precision},P{w.order}>" if mixed else
-                       f"rhs_{plan.kernel_variant}_kernel<{w.equation},P{w.order}>"),
+            "traffic": measured_traffic(w.name, kernel_name),
+            "traffic_source": "profiles/hbm_traffic_fp64.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+            "kernel": kernel_name,
             "kernel_us": kernel_ms * 1e3,
             "algorithmic_bytes": b_alg,
             "algorithmic_flops": f_alg,
