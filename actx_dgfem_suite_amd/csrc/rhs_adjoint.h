@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
       }
       const int64_t out = e0 * NP + tid;
 #pragma unroll
-      for (int c = 0; c < NF; ++c) a.qbar[c][out] = (T)acc[c];
+      for (int c = 0; c < NF; ++c) __builtin_nontemporal_store((T)acc[c], a.qbar[c] + out);
     }
   }
 }

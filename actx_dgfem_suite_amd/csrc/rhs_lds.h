@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       }
       const int64_t out = e0 * NP + tid;
 #pragma unroll
-      for (int c = 0; c < NF; ++c) a.rhs.p[c][out] = acc[c];
+      for (int c = 0; c < NF; ++c) __builtin_nontemporal_store(acc[c], a.rhs.p[c] + out);
     }
   }
 }

@@ -322,6 +322,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
     // }}}
 
     // {{{ output, one field at a time through the buffer to contiguous stores
+    // (non-temporal: the RHS is not read again by this kernel, and keeping it
+    // out of L2 leaves room for the state the neighbours' gathers reuse)
 
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
       wave_lds_fence();
       double* dst = a.rhs.p[c] + tile * 16 * NP;
       for (int idx = lane; idx < 16 * NP; idx += 64)
-        if (tile * 16 + idx / NP < a.nelem) dst[idx] = buf[idx];
+        if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
       wave_lds_fence();
     }
 

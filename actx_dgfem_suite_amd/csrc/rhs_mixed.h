@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       float* dst = a.rhs[c] + tile * 16 * NP;
       for (int idx = lane; idx < 16 * NP; idx += 64)
-        if (tile * 16 + idx / NP < a.nelem) dst[idx] = buf[idx];
+        if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     }
 
