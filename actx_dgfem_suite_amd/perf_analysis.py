@@ -91,14 +91,25 @@ class MeshCounts:
         return 4 * self.k * self.nfp
 
 
+def box_mesh_counts(degree: int, nel_1d: int) -> MeshCounts:
+    """Sizes of the box mesh of ``nel_1d``^3 hexahedra x 6 tetrahedra."""
+    n = int(nel_1d)
+    return MeshCounts(k=6 * n ** 3, np=np_for_order(degree), nfp=nfp_for_order(degree),
+                      nbdry=12 * n * n)
+
+
 def mesh_counts(equation: str, dim: int, degree: int) -> MeshCounts:
     """Sizes of the reference tier's box mesh (rhs_builder.py:9-45)."""
     if dim != 3:
         raise NotImplementedError("only 3D operators are built")
     ndofs, _ = parse_equation(equation)
-    n = get_nel_1d_for_regular_rect_mesh(dim, degree, ndofs)
-    return MeshCounts(k=6 * n ** 3, np=np_for_order(degree), nfp=nfp_for_order(degree),
-                      nbdry=12 * n * n)
+    return box_mesh_counts(degree, get_nel_1d_for_regular_rect_mesh(dim, degree, ndofs))
+
+
+def float64_flops_on_mesh(equation: str, m: MeshCounts) -> int:
+    """The counting rule applied to ``equation`` on the mesh sizes ``m``."""
+    _, base = parse_equation(equation)
+    return {"wave": _wave_flops, "euler": _euler_flops, "maxwell": _maxwell_flops}[base](m)
 
 
 def _contractions(m: MeshCounts, n_div: int, n_grad: int, n_lift: int) -> int:
@@ -168,9 +179,7 @@ def _maxwell_flops(m: MeshCounts) -> int:
 
 @cache
 def get_float64_flops(equation: str, dim: int, degree: int) -> int:
-    m = mesh_counts(equation, dim, degree)
-    _, base = parse_equation(equation)
-    return {"wave": _wave_flops, "euler": _euler_flops, "maxwell": _maxwell_flops}[base](m)
+    return float64_flops_on_mesh(equation, mesh_counts(equation, dim, degree))
 
 
 @cache

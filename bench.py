@@ -41,6 +41,11 @@ def _parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--adjoint-steps", type=int, default=200,
                    help="timed VJP evaluations for the adjoint/forward cost ratio (0: skip)")
+    p.add_argument("--trajectory-adjoint", action="store_true",
+                   help="instead of the RHS benchmark: gradient of a final-time misfit through "
+                        "an LSERK4 trajectory (maxwell_p4: FP64, 100 steps, checkpoints every 10; "
+                        "wave_p4_large: BF16x3 forward, FP32 adjoint, 200 steps, every 20, "
+                        "error against FP64 forward + adjoint)")
     p.add_argument("--copy-probe", action="store_true",
                    help="also run a plain device copy of known size (HBM counter "
                         "calibration under rocprofv3 --pmc)")
@@ -111,6 +116,52 @@ def cpu_baseline(workload, seconds: float):
     }
 
 
+def trajectory_adjoint(w, plan, d, q, dev) -> dict:
+    """Forward trajectory alone, then forward + reverse sweep, timed on the
+    host around device synchronisations; the mixed wave workload also runs the
+    FP64 forward + adjoint and reports the gradient's relative L2 error."""
+    import numpy as np
+    import torch
+
+    from actx_dgfem_suite_amd.timestepping import LSERK4, cfl_timestep, misfit_gradient
+    from actx_dgfem_suite_amd.workloads import initial_state, maxwell_target
+
+    if w.name == "maxwell_p4":
+        nsteps, every, fwd, adj = 100, 10, "fp64", torch.float64
+        target = [torch.from_numpy(np.ascontiguousarray(f)).to(dev) for f in maxwell_target(d)]
+    elif w.name == "wave_p4_large":
+        nsteps, every, fwd, adj = 200, 20, "bf16x3", torch.float32
+        target = [torch.from_numpy(np.ascontiguousarray(initial_state(w, d, seed=1)[0])).to(dev)]
+    else:
+        nsteps, every, fwd, adj = 100, 10, "fp64", torch.float64
+        target = [t.clone() for t in q]
+    speed = 2.0 if w.equation == "euler" else 1.0
+    dt = cfl_timestep(d, speed)
+    st = LSERK4(plan, dt, precision=fwd)
+    qf = [t.clone() for t in q]
+    st.run(qf, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st.run(qf, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    j, grad = misfit_gradient(st, q, target, nsteps, every, adjoint_dtype=adj)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t0
+    out = {"workload": w.name, "steps": nsteps, "checkpoint_every": every,
+           "forward_precision": fwd, "adjoint_dtype": str(adj).replace("torch.", ""),
+           "dt": dt, "objective": j, "forward_s": t_fwd, "forward_plus_adjoint_s": t_all,
+           "adjoint_over_forward": (t_all - t_fwd) / t_fwd}
+    if fwd != "fp64":
+        j64, g64 = misfit_gradient(LSERK4(plan, dt), q, target, nsteps, every)
+        out["objective_fp64"] = j64
+        out["grad_rel_l2_vs_fp64"] = [
+            float(torch.linalg.norm(a.double() - b) / max(float(torch.linalg.norm(b)), 1e-300))
+            for a, b in zip(grad, g64)]
+    return out
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -121,6 +172,7 @@ def main():
     import numpy as np
     import torch
 
+    from actx_dgfem_suite_amd.perf_analysis import box_mesh_counts, float64_flops_on_mesh
     from actx_dgfem_suite_amd.plan import DGPlan
     from actx_dgfem_suite_amd.workloads import (
         WORKLOADS,
@@ -151,6 +203,10 @@ def main():
                            / max(float(torch.linalg.norm(r)), 1e-3 * total))
                      for g, r in zip(got, ref)]
         del ref, got
+    if args.trajectory_adjoint:
+        print(json.dumps({"trajectory_adjoint": trajectory_adjoint(w, plan, d, q, dev)}),
+              flush=True)
+        return
     out = plan.empty_fields(torch.float32 if mixed else torch.float64)
     nf = len(q)
     k, npn, nfp = d.nelements, d.np, d.nfp
@@ -211,6 +267,8 @@ def main():
 
     b_alg = algorithmic_bytes(k, npn, nfp, nf, state_bytes=4 if mixed else 8)
     f_alg = algorithmic_flops(w.equation, k, npn, nfp, nf)
+    # the reference's perf_analysis counting rule on this mesh
+    rule_flops = float64_flops_on_mesh(w.equation, box_mesh_counts(w.order, w.nel_1d))
     t_kernel = kernel_ms * 1e-3
     achieved = b_alg / t_kernel / 1e9
     t_roof = max(b_alg / (HBM_PEAK_GBS * 1e9), f_alg / (FP64_PEAK_TFLOPS * 1e12))
@@ -243,6 +301,7 @@ def main():
             "variant": args.variant,
         },
         "gflops": f_alg / t_kernel / 1e9,
+        "gflops_perf_analysis": rule_flops / t_kernel / 1e9,
         "gdofs_per_s": k * npn / t_kernel / 1e9,
         "roofline": {
             "bound": "hbm",
