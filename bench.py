@@ -133,9 +133,10 @@ def trajectory_adjoint(w, plan, d, q, dev) -> dict:
         nsteps, every, fwd, adj = 200, 20, "bf16x3", torch.float32
         target = [torch.from_numpy(np.ascontiguousarray(initial_state(w, d, seed=1)[0])).to(dev)]
     else:
-        nsteps, every, fwd, adj = 100, 10, "fp64", torch.float64
-        target = [t.clone() for t in q]
-    speed = 2.0 if w.equation == "euler" else 1.0
+        raise SystemExit(f"--trajectory-adjoint is defined for maxwell_p4 and wave_p4_large, "
+                         f"not {w.name} (the Euler benchmark state, a uniform flow into walls, "
+                         "forms a shock that an unlimited DG solution cannot follow)")
+    speed = 1.0
     dt = cfl_timestep(d, speed)
     st = LSERK4(plan, dt, precision=fwd)
     qf = [t.clone() for t in q]
