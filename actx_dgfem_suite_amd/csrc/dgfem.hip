@@ -850,9 +850,15 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     };
     p->vjp_grid = grid_of((const void*)p->vjp_kernel);
     p->vjp_grid32 = grid_of((const void*)p->vjp_kernel32);
-    int64_t mg = (int64_t)prop.multiProcessorCount * 8;
-    const int64_t mneed = (k * np + kBlock - 1) / kBlock;
-    p->massinv_grid = (int)(mg < mneed ? mg : mneed);
+    {
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->massinv_kernel,
+                                                        kBlock, 0) != hipSuccess ||
+          occ < 1)
+        occ = 1;
+      int64_t mg = (int64_t)prop.multiProcessorCount * occ;
+      p->massinv_grid = (int)(mg < vbatch ? mg : vbatch);
+    }
     p->has_vjp = true;
   }
   *out = p;

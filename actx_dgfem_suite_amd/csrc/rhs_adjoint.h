@@ -210,28 +210,36 @@ template <> struct AdjOf<DG_EQ_MAXWELL> { using type = MaxwellAdj; };
 
 // }}}
 
-// {{{ mu = M^{-1} lam per element: one thread per (element, node), the
-// element's lam row read straight from memory (the threads of one element
-// share its cache lines), M^{-1} in LDS column-major so lanes read
-// consecutive words
+// {{{ mu = M^{-1} lam per element: batches of kBlock / Np elements; every
+// field of a batch is staged in LDS with coalesced loads, then one thread per
+// (element, node) forms its row of M^{-1} lam and stores it coalesced.  A
+// persistent grid of several blocks per CU keeps loads of many batches in
+// flight.
 
 template <int ORDER, typename T>
 __global__ __launch_bounds__(kBlock) void mass_inv_kernel(MassInvArgs<T> a) {
   constexpr int NP = np_of(ORDER);
+  constexpr int EB = kBlock / NP;
   __shared__ double s_m[NP * NP];     // s_m[j * NP + i] = Minv[i][j] (= Minv[j][i])
-  for (int k = threadIdx.x; k < NP * NP; k += kBlock) s_m[k] = a.mass_inv[k];
-  __syncthreads();
-  const int64_t total = a.nelem * NP;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < total; idx += stride) {
-    const int64_t e = idx / NP;
-    const int i = (int)(idx - e * NP);
-    for (int c = 0; c < a.nf; ++c) {
-      const T* row = a.lam[c] + e * NP;
-      double acc = 0.0;
+  __shared__ double s_l[kMaxFields][EB * NP];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < NP * NP; k += kBlock) s_m[k] = a.mass_inv[k];
+  const int el = tid / NP, i = tid - el * NP;
+  const int64_t nbatch = (a.nelem + EB - 1) / EB;
+  for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+    const int64_t e0 = b * EB;
+    const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
+    __syncthreads();
+    for (int c = 0; c < a.nf; ++c)
+      for (int k = tid; k < nel * NP; k += kBlock) s_l[c][k] = (double)a.lam[c][e0 * NP + k];
+    __syncthreads();
+    if (el < nel) {
+      for (int c = 0; c < a.nf; ++c) {
+        double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < NP; ++j) acc += s_m[j * NP + i] * (double)row[j];
-      a.mu[c][idx] = (T)acc;
+        for (int j = 0; j < NP; ++j) acc += s_m[j * NP + i] * s_l[c][el * NP + j];
+        a.mu[c][e0 * NP + tid] = (T)acc;
+      }
     }
   }
 }
@@ -274,16 +282,64 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
   for (int k = tid; k < NP * 3; k += kBlock) s_nf[k] = a.node_faces[k];
 
   const int32_t p_end = a.sched.part[block_xcd() + 1];
+  const int32_t p_step = blocks_per_xcd();
   const int el = tid / NP;
   const int i = tid - el * NP;
   const double c0 = a.p0;
 
-  for (int32_t pos = a.sched.part[block_xcd()] + block_in_xcd(); pos < p_end;
-       pos += blocks_per_xcd()) {
+  // The exterior gathers of the next batch are issued into registers while
+  // this batch is processed, so their latency overlaps the face and volume
+  // phases.  R face-node slots per thread cover the batch's EB * NFN nodes.
+  constexpr int R = (EB * NFN + kBlock - 1) / kBlock;
+  int32_t pf_ext[R];
+  double pf_mu[R][NF];
+  double pf_q[R][NQ];
+  auto batch_nel = [&](int64_t bb) {
+    const int64_t e = bb * EB;
+    return (int)(a.nelem - e < EB ? a.nelem - e : EB);
+  };
+  auto issue_map = [&](int64_t bb) {
+    const int n = batch_nel(bb) * NFN;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + r * kBlock;
+      pf_ext[r] = k < n ? a.emap[bb * EB * NFN + k] : -1;
+    }
+  };
+  auto issue_gathers = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
+        pf_mu[r][c] = pf_ext[r] >= 0 ? (double)a.mu[c][pf_ext[r]] : 0.0;
+        if constexpr (NEEDS_Q) pf_q[r][c] = pf_ext[r] >= 0 ? (double)a.q[c][pf_ext[r]] : 0.0;
+      }
+  };
+
+  int32_t pos = a.sched.part[block_xcd()] + block_in_xcd();
+  if (pos < p_end) {
+    issue_map(a.sched.order[pos]);
+    issue_gathers();
+  }
+  for (; pos < p_end; pos += p_step) {
     const int64_t b = a.sched.order[pos];
     const int64_t e0 = b * EB;
-    const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
+    const int nel = batch_nel(b);
     __syncthreads();
+    // this batch's exterior values, gathered during the previous batch
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + r * kBlock;
+      if (k < nel * NFN) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) {
+          s_mux[c][k] = pf_mu[r][c];
+          if constexpr (NEEDS_Q) s_x[c][k] = pf_q[r][c];
+        }
+      }
+    }
+    const bool has_next = pos + p_step < p_end;
+    if (has_next) issue_map(a.sched.order[pos + p_step]);
 #pragma unroll
     for (int c = 0; c < NF; ++c)
       for (int k = tid; k < nel * NP; k += kBlock) {
@@ -292,15 +348,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
       }
     for (int k = tid; k < nel * kGeo; k += kBlock) s_geo[k] = a.geo[e0 * kGeo + k];
     for (int k = tid; k < nel * kFaces; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * kFaces + k];
-    // exterior gathers
-    for (int k = tid; k < nel * NFN; k += kBlock) {
-      const int32_t ext = a.emap[e0 * NFN + k];
-#pragma unroll
-      for (int c = 0; c < NF; ++c) {
-        s_mux[c][k] = ext >= 0 ? (double)a.mu[c][ext] : 0.0;
-        if constexpr (NEEDS_Q) s_x[c][k] = ext >= 0 ? (double)a.q[c][ext] : 0.0;
-      }
-    }
+    if (has_next) issue_gathers();
     __syncthreads();
 
     const bool active = el < nel;
