@@ -38,15 +38,45 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
   for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
 
   const int32_t p_end = a.sched.part[block_xcd() + 1];
+  const int32_t p_step = blocks_per_xcd();
   const int el = tid / NP;
   const int i = tid - el * NP;
   const double c0 = a.p0;
 
-  for (int32_t pos = a.sched.part[block_xcd()] + block_in_xcd(); pos < p_end;
-       pos += blocks_per_xcd()) {
+  // Exterior values of the next batch are gathered into registers during
+  // this batch's volume phase; R face-node slots per thread cover a batch.
+  constexpr int R = (EB * NFN + kBlock - 1) / kBlock;
+  int32_t pf_ext[R];
+  double pf_q[R][NF];
+  auto batch_nel = [&](int64_t bb) {
+    const int64_t e = bb * EB;
+    return (int)(a.nelem - e < EB ? a.nelem - e : EB);
+  };
+  auto issue_map = [&](int64_t bb) {
+    const int n = batch_nel(bb) * NFN;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + r * kBlock;
+      pf_ext[r] = k < n ? a.emap[bb * EB * NFN + k] : -1;
+    }
+  };
+  auto issue_gathers = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < NF; ++c) pf_q[r][c] = pf_ext[r] >= 0 ? a.q.p[c][pf_ext[r]] : 0.0;
+  };
+
+  int32_t pos = a.sched.part[block_xcd()] + block_in_xcd();
+  if (pos < p_end) {
+    issue_map(a.sched.order[pos]);
+    issue_gathers();
+  }
+  for (; pos < p_end; pos += p_step) {
     const int64_t b = a.sched.order[pos];
     const int64_t e0 = b * EB;
-    const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
+    const int nel = batch_nel(b);
+    const bool has_next = pos + p_step < p_end;
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
@@ -87,8 +117,11 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
             s_w[c * 3 + r][node] = g[r] * f[c][0] + g[3 + r] * f[c][1] + g[6 + r] * f[c][2];
       }
     }
-    // face fluxes, scaled by Fscale
-    for (int k = tid; k < nel * NFN; k += kBlock) {
+    // face fluxes, scaled by Fscale (exterior values from this thread's slots)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + r * kBlock;
+      if (k >= nel * NFN) continue;
       const int fe = k / NFN;
       const int fn = k - fe * NFN;
       const int face = fn / NFP;
@@ -98,16 +131,19 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       const int vn = fe * NP + s_fmask[fn];
 #pragma unroll
       for (int c = 0; c < NF; ++c) qm[c] = s_q[c][vn];
-      const int32_t ext = a.emap[(e0 + fe) * NFN + fn];
-      if (ext >= 0) {
+      if (pf_ext[r] >= 0) {
 #pragma unroll
-        for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
+        for (int c = 0; c < NF; ++c) qp[c] = pf_q[r][c];
       } else {
         Flux::boundary(qm, n, qp);
       }
       Flux::flux(qm, qp, n, c0, out);
 #pragma unroll
       for (int c = 0; c < NF; ++c) s_fl[c][fe * NFN + fn] = g[3] * out[c];
+    }
+    if (has_next) {
+      issue_map(a.sched.order[pos + p_step]);
+      issue_gathers();
     }
     __syncthreads();
 
