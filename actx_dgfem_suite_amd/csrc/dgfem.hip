@@ -182,13 +182,13 @@ MixedKernel pick_mixed(int prec, int eq, int order) {
   return nullptr;
 }
 
-// The FP64 kernel "auto" picks, from measurements on MI355X at 1M-4M DOFs per
-// field: the MFMA kernel, except for Maxwell at P2-P4, whose 18
-// reference-derivative contractions per element make the MFMA kernel
-// register-bound (one wave per SIMD, spills at P4) while the LDS-staged
-// kernel keeps 2-3 waves per SIMD.
-int auto_variant(int equation, int order) {
-  return (equation == DG_EQ_MAXWELL && order >= 2) ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
+// The FP64 kernel "auto" picks, from measurements on MI355X at 4M DOFs per
+// field: the MFMA kernel for wave and Euler, the LDS-staged kernel for
+// Maxwell, whose 18 reference-derivative contractions per element make the
+// MFMA kernel register-bound (one wave per SIMD from P2, spills at P4) while
+// the LDS-staged kernel keeps 2-3 waves per SIMD (P1: within 3%).
+int auto_variant(int equation, int /*order*/) {
+  return equation == DG_EQ_MAXWELL ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
 }
 
 // {{{ locality schedule
