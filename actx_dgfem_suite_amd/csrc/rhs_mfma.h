@@ -59,6 +59,15 @@ struct MfmaShape {
   static constexpr int LIFT_TILES = MB * KSL;
 };
 
+// Whether a wave loads the exterior-map entries of its next flux chunk ahead
+// of time.  It costs C registers per lane; measured on MI355X, it pays where
+// it leaves the occupancy unchanged (Euler P2 -2%, wave P4 -6%) and does not
+// for wave P1 and P3, where those registers cost a wave per SIMD (wave P3 +11%).
+template <int EQ, int ORDER>
+constexpr bool prefetch_ext() {
+  return !(EQ == DG_EQ_WAVE && (ORDER == 1 || ORDER == 3));
+}
+
 // LIFT k-steps whose flux one wave stages in LDS at a time.
 template <int EQ, int ORDER>
 constexpr int flux_chunk() {
@@ -129,8 +138,27 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   const double c0 = a.p0;
   double* const buf = s_buf[tid >> 6];
 
+  // exterior-map entries of one flux chunk's items, loaded one chunk ahead
+  // (the first chunk's before the volume phase) so that the gathers do not
+  // wait on the map
+  constexpr bool PF = prefetch_ext<EQ, ORDER>();
+  int32_t ext_pf[C];
+  auto load_ext = [&](int64_t t, int ch) {
+#pragma unroll
+    for (int it = 0; it < C; ++it) {
+      const int item = it * 64 + lane;
+      const int kk = item % C;
+      const int p = item / C;
+      const int ks = ch * C + kk;
+      int64_t pe = t * 16 + (p & 15);
+      pe = pe < a.nelem ? pe : a.nelem - 1;
+      ext_pf[it] = ks < NFP ? a.emap[pe * NFN + (p >> 4) * NFP + ks] : 0;
+    }
+  };
+
   for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
     const int64_t tile = a.sched.order[pos];
+    if constexpr (PF) load_ext(tile, 0);
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
@@ -289,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           double qm[NF], qp[NF], out[NF];
 #pragma unroll
           for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
-          const int32_t ext = a.emap[pe * NFN + fn];
+          const int32_t ext = PF ? ext_pf[it] : a.emap[pe * NFN + fn];
           if (ext >= 0) {
 #pragma unroll
             for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
@@ -301,6 +329,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           for (int c = 0; c < NF; ++c) buf[(kk * NF + c) * FROW + p] = fscale * out[c];
         }
       }
+      if constexpr (PF)
+        if (ch + 1 < NCHUNK) load_ext(tile, ch + 1);
       wave_lds_fence();
       // consumer
 #pragma unroll
