@@ -84,3 +84,11 @@ def test_mixed_context_keeps_float32_state(precision, device):
     ref = oracle_rhs(op.equation, od, q, op.param)
     got = np.stack(flatten_fields(actx.to_numpy(out))).astype(np.float64)
     assert max(field_errors(got, ref)) < 1e-4
+
+
+def test_dof_scaling_tool_runs(device, capsys):
+    from actx_dgfem_suite_amd.tools.wave_dof_scaling import main
+    print("[case] dof scaling tool", flush=True)
+    t = main(dim=3, degrees=[1, 2], ndofs_list=[1000, 20000])
+    assert t.shape == (2, 2) and np.all(t > 0)
+    assert "GDOFS/s for 3D-wave" in capsys.readouterr().out

@@ -100,3 +100,10 @@ def test_runner_command_line():
     assert a.no_verify and a.equations == "wave,tiny_euler"
     assert set(_NAME_TO_ACTX_CLASS) == {"hip:dgfem", "hip:dgfem_lds", "hip:dgfem_fp32",
                                          "hip:dgfem_bf16x3"}
+
+
+def test_dof_scaling_command_line():
+    from actx_dgfem_suite_amd.tools.wave_dof_scaling import DEFAULT_NDOFS, _parse
+    a = _parse(["--dim", "3", "--degrees", "2,3", "--ndofs", "1e6,4e6"])
+    assert a.dim == 3 and a.degrees == "2,3" and a.ndofs == "1e6,4e6"
+    assert _parse([]).ndofs == ",".join(str(n) for n in DEFAULT_NDOFS)
