@@ -34,6 +34,13 @@ Independent routes used here (so that a shared mistake is unlikely):
 * exterior face nodes by nearest-neighbour search on physical coordinates
   (``scipy.spatial.cKDTree``) instead of topological vertex permutations.
 
+Triangles (2D) take the same routes: ``int xi^a eta^b = a! b! / (a+b+2)!``
+over the unit triangle, edge mass matrices from the exact interval moments
+``1/(a+b+1)``, outward normals by rotating edge vectors away from the element
+centroid, twin edges by coordinates.  The operators are written over
+``range(dim)``; 2D Maxwell is the TM restriction (E_z, H_x, H_y) of the 3D
+operator.
+
 Inputs shared with the engine are only the mesh (vertex coordinates and the
 element-vertex table) and the reference node set (the warp-and-blend
 convention); both are pinned separately by closed-form tests.
@@ -68,11 +75,12 @@ def _monomial_exponents(order, dim):
 
 @dataclass(frozen=True)
 class OracleRefElement:
-    nodes: np.ndarray        # (Np, 3) biunit coordinates
-    mass: np.ndarray         # (Np, Np) on the biunit tet
-    diff: np.ndarray         # (3, Np, Np) d/dr, d/ds, d/dt
-    face_nodes: list         # 4 arrays of volume-node indices
-    face_mass_unit: list     # 4 (Nfp, Nfp): int over the unit triangle (area 1/2)
+    nodes: np.ndarray        # (Np, dim) biunit coordinates
+    mass: np.ndarray         # (Np, Np) on the biunit simplex
+    diff: np.ndarray         # (dim, Np, Np) d/dr, d/ds(, d/dt)
+    face_nodes: list         # dim+1 arrays of volume-node indices
+    face_mass_unit: list     # (Nfp, Nfp) per face: int over the unit triangle
+                             # (area 1/2) in 3D, the unit interval in 2D
 
 
 def _mp_matrix(rows):
@@ -162,18 +170,93 @@ def oracle_reference_element(nodes):
 # }}}
 
 
+# local vertex pairs of the reference triangle's edges (s=-1, r+s=0, r=-1)
+EDGE_VERTICES = ((0, 1), (1, 2), (2, 0))
+REF_VERTS_2D = np.array([[-1., -1.], [1., -1.], [-1., 1.]])
+
+
+def oracle_reference_triangle(nodes):
+    """Monomial-basis nodal operators on a biunit-triangle node set, as
+    :func:`oracle_reference_element` does for the tetrahedron: closed-form
+    integrals over the unit triangle (int xi^a eta^b = a! b! / (a+b+2)!) and
+    the unit interval for the edges, in 40-digit arithmetic.  ``diff`` has 2
+    rows; ``face_mass_unit`` integrates over each edge parametrised on [0, 1]."""
+    import mpmath
+    key = ("tri", nodes.tobytes())
+    if key in _REF_CACHE:
+        return _REF_CACHE[key]
+    mpmath.mp.dps = 40
+    npn = nodes.shape[0]
+    order = 0
+    while (order + 1) * (order + 2) // 2 < npn:
+        order += 1
+    exps = _monomial_exponents(order, 2)
+    assert len(exps) == npn
+    xi = [[(mpmath.mpf(float(c)) + 1) / 2 for c in row] for row in nodes]
+    v = _mp_matrix([[xi[i][0] ** e[0] * xi[i][1] ** e[1] for e in exps] for i in range(npn)])
+    vinv = v ** -1
+    diff = []
+    for d in range(2):
+        vd = mpmath.matrix(npn, npn)
+        for i in range(npn):
+            for k, e in enumerate(exps):
+                if e[d] == 0:
+                    continue
+                ee = list(e)
+                ee[d] -= 1
+                vd[i, k] = e[d] * xi[i][0] ** ee[0] * xi[i][1] ** ee[1]
+        diff.append(_to_np(vd * vinv) * 0.5)
+    mm = _mp_matrix([[mpmath.factorial(e1[0] + e2[0]) * mpmath.factorial(e1[1] + e2[1])
+                      / mpmath.factorial(e1[0] + e2[0] + e1[1] + e2[1] + 2)
+                      for e2 in exps] for e1 in exps])
+    mass = _to_np(vinv.T * mm * vinv) * 4.0        # biunit area = 4 * unit area
+    face_nodes, face_mass = [], []
+    nfp = order + 1
+    m1 = _mp_matrix([[mpmath.mpf(1) / (a + b + 1) for b in range(nfp)] for a in range(nfp)])
+    for a_, b_ in EDGE_VERTICES:
+        a0, a1 = REF_VERTS_2D[a_], REF_VERTS_2D[b_]
+        tang = a1 - a0
+        rel = nodes - a0
+        cross = rel[:, 0] * tang[1] - rel[:, 1] * tang[0]
+        idx = np.nonzero(np.abs(cross) < 1e-10)[0]
+        face_nodes.append(idx)
+        # edge coordinate in [0, 1], exact from the node coordinates
+        k = 0 if abs(tang[0]) > 0 else 1
+        ts = [(mpmath.mpf(float(nodes[i][k])) - float(a0[k])) / float(tang[k]) for i in idx]
+        v1 = _mp_matrix([[t ** e for e in range(nfp)] for t in ts])
+        v1inv = v1 ** -1
+        face_mass.append(_to_np(v1inv.T * m1 * v1inv))
+    out = OracleRefElement(nodes=nodes, mass=mass, diff=np.stack(diff),
+                           face_nodes=face_nodes, face_mass_unit=face_mass)
+    _REF_CACHE[key] = out
+    return out
+
+# }}}
+
+
 # {{{ discretization
 
 @dataclass
 class OracleDiscretization:
     ref: OracleRefElement
-    x: np.ndarray          # (3, K, Np) physical nodes
-    rx: np.ndarray         # (K, 3(x), 3(r)) d r / d x
+    x: np.ndarray          # (dim, K, Np) physical nodes
+    rx: np.ndarray         # (K, dim(x), dim(r)) d r / d x
     jac: np.ndarray        # (K,)
-    normals: np.ndarray    # (K, 4, 3)
-    area: np.ndarray       # (K, 4) physical face areas
-    ext_index: np.ndarray  # (K, 4, Nfp) flat exterior volume-node index, -1 on boundary
+    normals: np.ndarray    # (K, nfaces, dim)
+    area: np.ndarray       # (K, nfaces) physical face areas (edge lengths in 2D)
+    ext_index: np.ndarray  # (K, nfaces, Nfp) flat exterior volume-node index, -1 on boundary
     nelements: int
+    # physical measure of a face per unit of the reference face measure the
+    # face_mass_unit matrices use (3D: unit triangle of area 1/2; 2D: [0, 1])
+    face_measure: np.ndarray = None
+
+    @property
+    def dim(self):
+        return self.x.shape[0]
+
+    @property
+    def nfaces(self):
+        return self.dim + 1
 
     @property
     def np(self):
@@ -182,6 +265,65 @@ class OracleDiscretization:
     @property
     def nfp(self):
         return len(self.ref.face_nodes[0])
+
+
+def _twin_faces(own, xflat, tol):
+    """Exterior node index of every face node by matching faces with equal
+    centroids and then nodes with equal coordinates (k-d tree)."""
+    from scipy.spatial import cKDTree
+    nf_total, nfp = own.reshape(-1, own.shape[-1]).shape
+    fpts = xflat[own.reshape(-1, nfp)]                  # (K*F, Nfp, dim)
+    cent = fpts.mean(axis=1)
+    tree = cKDTree(cent)
+    dist, nn = tree.query(cent, k=2)
+    twin = np.full(cent.shape[0], -1, dtype=np.int64)
+    for col in range(2):
+        ok = (dist[:, col] < tol) & (nn[:, col] != np.arange(cent.shape[0]))
+        twin[ok] = nn[ok, col]
+    ext = np.full((nf_total, nfp), -1, dtype=np.int64)
+    has = np.nonzero(twin >= 0)[0]
+    a = fpts[has]
+    b = fpts[twin[has]]
+    dd = np.linalg.norm(a[:, :, None, :] - b[:, None, :, :], axis=3)
+    j2 = dd.argmin(axis=2)
+    if np.any(np.take_along_axis(dd, j2[:, :, None], axis=2) > tol):
+        raise RuntimeError("non-conforming face")
+    ext[has] = np.take_along_axis(own.reshape(-1, nfp)[twin[has]], j2, axis=1)
+    return ext.reshape(own.shape)
+
+
+def build_discretization_2d(vertices, elements, ref_nodes):
+    """Triangles: metric by nodal differentiation, normals by rotating edge
+    vectors away from the centroid, twin edges by coordinates."""
+    ref = oracle_reference_triangle(np.asarray(ref_nodes, dtype=np.float64))
+    k = elements.shape[0]
+    xv = vertices[elements]                             # (K, 3, 2)
+    r, s = ref.nodes.T
+    lam = np.stack([-(r + s) / 2, (1 + r) / 2, (1 + s) / 2], axis=1)
+    x = np.einsum("nv,evx->xen", lam, xv)               # (2, K, Np)
+    dxdr = np.einsum("rij,xej->erxi", ref.diff, x).mean(axis=3)   # (K, r, x)
+    jm = np.transpose(dxdr, (0, 2, 1))
+    jac = np.linalg.det(jm)
+    rx = np.transpose(np.linalg.inv(jm), (0, 2, 1))
+    centroid = xv.mean(axis=1)
+    normals = np.empty((k, 3, 2))
+    length = np.empty((k, 3))
+    for f, (i0, i1) in enumerate(EDGE_VERTICES):
+        t = xv[:, i1] - xv[:, i0]
+        ln = np.linalg.norm(t, axis=1)
+        n = np.stack([t[:, 1], -t[:, 0]], axis=1) / ln[:, None]
+        mid = 0.5 * (xv[:, i0] + xv[:, i1])
+        sgn = np.sign(np.einsum("ex,ex->e", n, mid - centroid))
+        normals[:, f] = n * sgn[:, None]
+        length[:, f] = ln
+    npn = ref.nodes.shape[0]
+    fidx = np.stack(ref.face_nodes)                     # (3, Nfp)
+    own = np.arange(k)[:, None, None] * npn + fidx[None]
+    tol = np.min(np.sqrt(np.abs(jac))) * 1e-6
+    ext = _twin_faces(own, x.reshape(2, -1).T, tol)
+    return OracleDiscretization(ref=ref, x=x, rx=rx, jac=jac, normals=normals,
+                                area=length, ext_index=ext, nelements=k,
+                                face_measure=length)
 
 
 def build_discretization(vertices, elements, ref_nodes):
@@ -234,7 +376,7 @@ def build_discretization(vertices, elements, ref_nodes):
     ext[has] = np.take_along_axis(own.reshape(-1, nfp)[twin[has]], j2, axis=1)
     return OracleDiscretization(ref=ref, x=x, rx=rx, jac=jac, normals=normals,
                                 area=area, ext_index=ext.reshape(k, NFACES, nfp),
-                                nelements=k)
+                                nelements=k, face_measure=2.0 * area)
 
 # }}}
 
@@ -245,7 +387,7 @@ def weak_local_d_dx(d: OracleDiscretization, axis, u):
     """int_E d(phi_i)/dx_axis u dx = J * sum_r (dr/dx) (S_r^T u), S_r = M D_r."""
     m = d.ref.mass
     out = np.zeros_like(u)
-    for r in range(3):
+    for r in range(d.dim):
         s_t = (m @ d.ref.diff[r]).T
         out += (d.jac * d.rx[:, axis, r])[:, None] * (u @ s_t.T)
     return out
@@ -253,7 +395,7 @@ def weak_local_d_dx(d: OracleDiscretization, axis, u):
 
 def local_d_dx(d: OracleDiscretization, axis, u):
     out = np.zeros_like(u)
-    for r in range(3):
+    for r in range(d.dim):
         out += d.rx[:, axis, r][:, None] * (u @ d.ref.diff[r].T)
     return out
 
@@ -263,11 +405,11 @@ def inverse_mass(d: OracleDiscretization, u):
 
 
 def face_mass(d: OracleDiscretization, flux):
-    """sum_f int_{face f} phi_i flux_f dA for flux of shape (K, 4, Nfp)."""
+    """sum_f int_{face f} phi_i flux_f dA for flux of shape (K, nfaces, Nfp)."""
     out = np.zeros((d.nelements, d.np))
-    for f in range(NFACES):
-        # physical measure = 2*area * (unit-triangle measure)
-        contrib = (flux[:, f, :] @ d.ref.face_mass_unit[f].T) * (2.0 * d.area[:, f])[:, None]
+    for f in range(d.nfaces):
+        # physical measure = face_measure * (unit reference-face measure)
+        contrib = (flux[:, f, :] @ d.ref.face_mass_unit[f].T) * d.face_measure[:, f][:, None]
         np.add.at(out, (slice(None), d.ref.face_nodes[f]), contrib)
     return out
 
@@ -296,8 +438,8 @@ def boundary_mask(d):
 # {{{ wave
 
 def wave_rhs(d: OracleDiscretization, u, v, c=1.0):
-    """wave_operator of equations/wave.py:121-200 (quad_tag=None)."""
-    n = d.normals[:, :, None, :]                        # (K, 4, 1, 3)
+    """wave_operator of equations/wave.py:121-200 (quad_tag=None), in 2D or 3D."""
+    n = d.normals[:, :, None, :]                        # (K, F, 1, dim)
     ui, ue = interior_trace(d, u), exterior_trace(d, u)
     vi = np.stack([interior_trace(d, vc) for vc in v], axis=-1)
     ve = np.stack([exterior_trace(d, vc) for vc in v], axis=-1)
@@ -311,10 +453,10 @@ def wave_rhs(d: OracleDiscretization, u, v, c=1.0):
     flux_u = c * (np.sum(v_avg * n, axis=-1) + 0.5 * u_diff)
     v_jump = np.sum(v_diff * n, axis=-1)
     flux_v = c * (u_avg[..., None] * n + 0.5 * v_jump[..., None] * n)
-    div_v = sum(weak_local_d_dx(d, i, v[i]) for i in range(3))
+    div_v = sum(weak_local_d_dx(d, i, v[i]) for i in range(d.dim))
     rhs_u = inverse_mass(d, -c * div_v + face_mass(d, flux_u))
     rhs_v = [inverse_mass(d, -c * weak_local_d_dx(d, i, u) + face_mass(d, flux_v[..., i]))
-             for i in range(3)]
+             for i in range(d.dim)]
     return rhs_u, rhs_v
 
 # }}}
@@ -323,19 +465,21 @@ def wave_rhs(d: OracleDiscretization, u, v, c=1.0):
 # {{{ euler
 
 def euler_volume_flux(q, gamma):
-    """Flux tensor F[field][x] of the conserved state (mass, energy, mom[3])."""
-    rho, e, mom = q[0], q[1], q[2:5]
+    """Flux tensor F[field][x] of the conserved state (mass, energy, mom[dim])."""
+    dim = len(q) - 2
+    rho, e, mom = q[0], q[1], q[2:2 + dim]
     u = mom / rho
     p = (gamma - 1.0) * (e - 0.5 * np.sum(mom * u, axis=0))
-    f_mass = [mom[i] for i in range(3)]
-    f_energy = [mom[i] * (e + p) / rho for i in range(3)]
-    f_mom = [[mom[i] * mom[j] / rho + (p if i == j else 0.0) for j in range(3)]
-             for i in range(3)]
+    f_mass = [mom[i] for i in range(dim)]
+    f_energy = [mom[i] * (e + p) / rho for i in range(dim)]
+    f_mom = [[mom[i] * mom[j] / rho + (p if i == j else 0.0) for j in range(dim)]
+             for i in range(dim)]
     return [f_mass, f_energy] + f_mom
 
 
 def _euler_wavespeed(q, gamma):
-    rho, e, mom = q[0], q[1], q[2:5]
+    dim = len(q) - 2
+    rho, e, mom = q[0], q[1], q[2:2 + dim]
     u = mom / rho
     p = (gamma - 1.0) * (e - 0.5 * np.sum(mom * u, axis=0))
     return np.sqrt(np.sum(u * u, axis=0)) + np.sqrt(gamma * (p / rho))
@@ -343,33 +487,37 @@ def _euler_wavespeed(q, gamma):
 
 def euler_numerical_flux(qi, qe, n, gamma):
     """LF flux {F}.n + lam/2 (q- - q+), lam = max of |u|+c over both sides."""
+    dim = len(qi) - 2
     fi = euler_volume_flux(qi, gamma)
     fe = euler_volume_flux(qe, gamma)
     lam = np.maximum(_euler_wavespeed(qi, gamma), _euler_wavespeed(qe, gamma))
     out = []
-    for c in range(5):
-        avg_n = sum(0.5 * (fi[c][x] + fe[c][x]) * n[..., x] for x in range(3))
+    for c in range(dim + 2):
+        avg_n = sum(0.5 * (fi[c][x] + fe[c][x]) * n[..., x] for x in range(dim))
         out.append(avg_n - 0.5 * lam * (qe[c] - qi[c]))
     return np.stack(out)
 
 
 def euler_rhs(d: OracleDiscretization, q, gamma=1.4):
-    """EulerOperator.operator with LF flux and inviscid walls (euler.py:168-179)."""
+    """EulerOperator.operator with LF flux and inviscid walls (euler.py:168-179),
+    in 2D or 3D."""
     q = np.asarray(q)
+    dim = d.dim
+    nfield = dim + 2
     flux = euler_volume_flux(q, gamma)
-    vol = np.stack([sum(weak_local_d_dx(d, x, flux[c][x]) for x in range(3))
-                    for c in range(5)])
-    qi = np.stack([interior_trace(d, q[c]) for c in range(5)])
-    qe = np.stack([exterior_trace(d, q[c]) for c in range(5)])
+    vol = np.stack([sum(weak_local_d_dx(d, x, flux[c][x]) for x in range(dim))
+                    for c in range(nfield)])
+    qi = np.stack([interior_trace(d, q[c]) for c in range(nfield)])
+    qe = np.stack([exterior_trace(d, q[c]) for c in range(nfield)])
     bd = boundary_mask(d)
-    n = d.normals                                        # (K, 4, 3)
-    nn = np.moveaxis(n, -1, 0)[:, :, :, None]            # (3, K, 4, 1)
-    mom_i = qi[2:5]
+    n = d.normals                                        # (K, F, dim)
+    nn = np.moveaxis(n, -1, 0)[:, :, :, None]            # (dim, K, F, 1)
+    mom_i = qi[2:2 + dim]
     mdotn = np.sum(mom_i * nn, axis=0)
     wall = np.concatenate([qi[0:2], mom_i - 2.0 * mdotn[None] * nn])
     qe = np.where(bd[None], wall, qe)
     fstar = euler_numerical_flux(qi, qe, n[:, :, None, :], gamma)
-    return np.stack([inverse_mass(d, vol[c] - face_mass(d, fstar[c])) for c in range(5)])
+    return np.stack([inverse_mass(d, vol[c] - face_mass(d, fstar[c])) for c in range(nfield)])
 
 # }}}
 
@@ -403,6 +551,37 @@ def maxwell_rhs(d: OracleDiscretization, w, alpha=0.5):
     rhs_e = [curl_h[i] - inverse_mass(d, face_mass(d, flux_e[i])) for i in range(3)]
     rhs_h = [-curl_e[i] - inverse_mass(d, face_mass(d, flux_h[i])) for i in range(3)]
     return np.stack(rhs_e + rhs_h)
+
+
+def maxwell_rhs_2d(d: OracleDiscretization, w, alpha=0.5):
+    """2D MaxwellOperator (eps = mu = 1, flux_type alpha, PEC) in TM form: the
+    fields (E_z, H_x, H_y) of the 3D operator with E_x = E_y = H_z = 0 and
+    no z-dependence; the 3D flux formulas with n_z = 0."""
+    w = np.asarray(w)
+    ez, hx, hy = w
+    dez = [local_d_dx(d, x, ez) for x in range(2)]
+    dhx_dy = local_d_dx(d, 1, hx)
+    dhy_dx = local_d_dx(d, 0, hy)
+    wi = np.stack([interior_trace(d, w[c]) for c in range(3)])
+    we = np.stack([exterior_trace(d, w[c]) for c in range(3)])
+    bd = boundary_mask(d)
+    pec = np.concatenate([-wi[0:1], wi[1:3]])
+    we = np.where(bd[None], pec, we)
+    zero = np.zeros_like(wi[0])
+    n = [d.normals[:, :, None, 0], d.normals[:, :, None, 1], zero]
+    de = [zero, zero, we[0] - wi[0]]
+    dh = [we[1] - wi[1], we[2] - wi[2], zero]
+    n_x_dh = _cross(n, dh)
+    n_x_de = _cross(n, de)
+    n_x_n_x_de = _cross(n, n_x_de)
+    n_x_n_x_dh = _cross(n, n_x_dh)
+    flux_ez = -0.5 * n_x_dh[2] + 0.5 * alpha * n_x_n_x_de[2]
+    flux_hx = 0.5 * n_x_de[0] + 0.5 * alpha * n_x_n_x_dh[0]
+    flux_hy = 0.5 * n_x_de[1] + 0.5 * alpha * n_x_n_x_dh[1]
+    rhs_ez = dhy_dx - dhx_dy - inverse_mass(d, face_mass(d, flux_ez))
+    rhs_hx = -dez[1] - inverse_mass(d, face_mass(d, flux_hx))
+    rhs_hy = dez[0] - inverse_mass(d, face_mass(d, flux_hy))
+    return np.stack([rhs_ez, rhs_hx, rhs_hy])
 
 # }}}
 
