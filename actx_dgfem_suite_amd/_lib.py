@@ -53,7 +53,7 @@ class DgPlanDesc(ctypes.Structure):
         ("ext_map", ctypes.POINTER(ctypes.c_int32)),
         ("params", ctypes.c_double * 4),
         ("variant", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
         ("mass_inv", ctypes.POINTER(ctypes.c_double)),
         ("face_mass", ctypes.POINTER(ctypes.c_double)),
         ("centroids", ctypes.POINTER(ctypes.c_double)),

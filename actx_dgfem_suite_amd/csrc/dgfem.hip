@@ -1,5 +1,6 @@
-// libdgfem.so: DG-FEM right-hand sides (wave, Euler, Maxwell; 3D affine
-// tetrahedra, P1-P4) for gfx950, behind the C ABI declared in include/dgfem.h.
+// libdgfem.so: DG-FEM right-hand sides (wave, Euler, Maxwell; affine
+// tetrahedra and triangles, P1-P4) for gfx950, behind the C ABI declared in
+// include/dgfem.h.
 //
 // One fused kernel per (equation, order) evaluates the whole RHS of a batch of
 // elements: the volume contraction with the reference derivative matrices, the
@@ -10,8 +11,9 @@
 //   weak divergence  xre,rij,xej->ei   (mass_inverse_fuser.py:39-56)
 //   weak gradient    re,rij,ej->ei     (mass_inverse_fuser.py:57-74)
 //   face mass/LIFT   ifj,fe,fej->ei    (mass_inverse_fuser.py:21-38)
-// Two variants: the MFMA kernel (rhs_mfma.h, default) and an LDS-staged
-// one-thread-per-node kernel (rhs_lds.h) kept as the measured baseline.
+// Two tetrahedral variants: the MFMA kernel (rhs_mfma.h) and an LDS-staged
+// one-thread-per-node kernel (rhs_lds.h); the measured-faster one is the
+// default per equation.  Triangles run the LDS-staged kernel of rhs_2d.h.
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +30,7 @@
 
 #include "dgfem.h"
 #include "dg_common.h"
+#include "rhs_2d.h"
 #include "rhs_lds.h"
 #include "rhs_mfma.h"
 #include "rhs_mixed.h"
@@ -152,6 +155,30 @@ KernelInfo mfma_info() {
 
 DG_DEFINE_PICK(pick_lds, lds_info)
 DG_DEFINE_PICK(pick_mfma, mfma_info)
+
+template <typename T>
+using TriKernel = void (*)(Rhs2DArgs<T>);
+
+template <int EQ, typename T>
+TriKernel<T> pick_2d_order(int order) {
+  switch (order) {
+    case 1: return rhs_2d_kernel<EQ, 1, T>;
+    case 2: return rhs_2d_kernel<EQ, 2, T>;
+    case 3: return rhs_2d_kernel<EQ, 3, T>;
+    case 4: return rhs_2d_kernel<EQ, 4, T>;
+    default: return nullptr;
+  }
+}
+
+template <typename T>
+TriKernel<T> pick_2d(int eq, int order) {
+  switch (eq) {
+    case DG_EQ_WAVE: return pick_2d_order<DG_EQ_WAVE, T>(order);
+    case DG_EQ_EULER: return pick_2d_order<DG_EQ_EULER, T>(order);
+    case DG_EQ_MAXWELL: return pick_2d_order<DG_EQ_MAXWELL, T>(order);
+    default: return nullptr;
+  }
+}
 
 using MixedKernel = void (*)(MixedArgs);
 
@@ -525,6 +552,7 @@ void build_tiles(int np, int nfp, const double* op, const double* lift,
 struct dg_plan {
   int equation = 0;
   int order = 0;
+  int dim = 3;
   int np = 0;
   int nfp = 0;
   int nfields = 0;
@@ -534,6 +562,9 @@ struct dg_plan {
   int grid = 0;          // persistent grid of the RHS kernel
   int lserk_grid = 0;
   KernelInfo kernel;
+  // triangles: FP64 and FP32-storage forms of the 2D kernel
+  TriKernel<double> tri_kernel = nullptr;
+  TriKernel<float> tri_kernel32 = nullptr;
   int variant = 0;
   double* geo = nullptr;
   int32_t* emap = nullptr;
@@ -608,6 +639,18 @@ int upload(T** dst, const T* src, size_t count) {
   return DG_OK;
 }
 
+template <typename T>
+void fill_2d_args(const dg_plan* p, Rhs2DArgs<T>& a) {
+  a.sched = {p->sched_batch, p->part_batch};
+  a.geo = p->geo;
+  a.emap = p->emap;
+  a.dmat_t = p->dmat_t;
+  a.lift_t = p->lift_t;
+  a.fmask = p->fmask;
+  a.nelem = p->nelem;
+  a.p0 = p->params[0];
+}
+
 int check_ptrs(const void* const* ptrs, int n, const char* what) {
   if (!ptrs) return fail(DG_ERR_INVALID_ARGUMENT, std::string(what) + " is NULL");
   for (int k = 0; k < n; ++k)
@@ -631,6 +674,18 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
       (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
     return rc;
+  if (p->dim == 2) {
+    // triangles: FP32 state, every operation in FP64 (both precisions)
+    Rhs2DArgs<float> a{};
+    for (int c = 0; c < p->nfields; ++c) {
+      a.q[c] = in[c];
+      a.rhs[c] = out[c];
+    }
+    fill_2d_args(p, a);
+    hipLaunchKernelGGL(p->tri_kernel32, dim3(p->grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+    DG_HIP_CHECK(hipGetLastError());
+    return DG_OK;
+  }
   const int k = precision == DG_PREC_FP32 ? 0 : 1;
   MixedArgs a{};
   for (int c = 0; c < p->nfields; ++c) {
@@ -690,15 +745,27 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   if (d->order < 1 || d->order > 4)
     return fail(DG_ERR_UNSUPPORTED, "order " + std::to_string(d->order) + " not supported (1-4)");
   if (d->nelements < 1) return fail(DG_ERR_INVALID_ARGUMENT, "nelements must be >= 1");
-  const int np = np_of(d->order), nfp = nfp_of(d->order);
+  if (d->dim != 0 && d->dim != 2 && d->dim != 3)
+    return fail(DG_ERR_UNSUPPORTED, "dim " + std::to_string(d->dim) + " not supported (2 or 3)");
+  const bool tri = d->dim == 2;
+  const int np = tri ? np2_of(d->order) : np_of(d->order);
+  const int nfp = tri ? nfp2_of(d->order) : nfp_of(d->order);
+  const int nfaces = tri ? kFaces2 : kFaces;
+  const int ndir = tri ? 2 : 3;
+  const int geo_stride = tri ? kGeo2 : kGeo;
   if ((int64_t)d->nelements * np >= (int64_t)INT32_MAX ||
-      (int64_t)d->nelements * kFaces * nfp >= (int64_t)INT32_MAX)
+      (int64_t)d->nelements * nfaces * nfp >= (int64_t)INT32_MAX)
     return fail(DG_ERR_UNSUPPORTED, "mesh too large for 32-bit node indices");
   if (!d->d_weak || !d->d_strong || !d->lift || !d->face_nodes || !d->geometry || !d->ext_map)
     return fail(DG_ERR_INVALID_ARGUMENT, "NULL operator or mesh array in descriptor");
+  if (tri && d->variant == DG_KERNEL_MFMA)
+    return fail(DG_ERR_UNSUPPORTED, "MFMA kernel: tetrahedral plans only (dim 2 runs the "
+                                    "LDS-staged kernel)");
+  if (tri && (d->mass_inv || d->face_mass))
+    return fail(DG_ERR_UNSUPPORTED, "adjoint (mass_inv / face_mass): tetrahedral plans only");
 
   const int64_t k = d->nelements;
-  const int nfn = kFaces * nfp;
+  const int nfn = nfaces * nfp;
   // validate the exterior map on the host: every index in range
   for (int64_t t = 0; t < k * nfn; ++t) {
     const int64_t v = d->ext_map[t];
@@ -738,14 +805,22 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   if (!p) return fail(DG_ERR_OUT_OF_MEMORY, "host allocation failed");
   p->equation = d->equation;
   p->order = d->order;
+  p->dim = tri ? 2 : 3;
   p->np = np;
   p->nfp = nfp;
-  p->nfields = nfields_of(d->equation);
+  p->nfields = tri ? nfields2_of(d->equation) : nfields_of(d->equation);
   p->nelem = k;
   for (int t = 0; t < 4; ++t) p->params[t] = d->params[t];
-  p->variant = d->variant == DG_KERNEL_AUTO ? auto_variant(d->equation, d->order) : d->variant;
-  p->kernel = p->variant == DG_KERNEL_LDS ? pick_lds(d->equation, d->order)
-                                          : pick_mfma(d->equation, d->order);
+  if (tri) {
+    p->variant = DG_KERNEL_LDS;
+    p->tri_kernel = pick_2d<double>(d->equation, d->order);
+    p->tri_kernel32 = pick_2d<float>(d->equation, d->order);
+    p->kernel = {nullptr, kBlock, kBlock / np};
+  } else {
+    p->variant = d->variant == DG_KERNEL_AUTO ? auto_variant(d->equation, d->order) : d->variant;
+    p->kernel = p->variant == DG_KERNEL_LDS ? pick_lds(d->equation, d->order)
+                                            : pick_mfma(d->equation, d->order);
+  }
 
   int rc;
   if (hipGetDevice(&p->device) != hipSuccess) {
@@ -754,20 +829,25 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   }
   // transposed operators: dmat_t[r][j][i] = D[r][i][j]; lift_t[fn][i] = LIFT[i][fn]
   const double* dsrc = d->equation == DG_EQ_MAXWELL ? d->d_strong : d->d_weak;
-  std::vector<double> dt(3 * np * np), lt((size_t)nfn * np);
-  for (int r = 0; r < 3; ++r)
+  std::vector<double> dt((size_t)ndir * np * np), lt((size_t)nfn * np);
+  for (int r = 0; r < ndir; ++r)
     for (int i = 0; i < np; ++i)
       for (int j = 0; j < np; ++j) dt[(r * np + j) * np + i] = dsrc[(r * np + i) * np + j];
   for (int i = 0; i < np; ++i)
     for (int fn = 0; fn < nfn; ++fn) lt[fn * np + i] = d->lift[i * nfn + fn];
-  std::vector<double> op_tiles, lift_tiles;
-  build_tiles(np, nfp, dsrc, d->lift, op_tiles, lift_tiles);
-  if ((rc = upload(&p->op_tiles, op_tiles.data(), op_tiles.size())) ||
-      (rc = upload(&p->lift_tiles, lift_tiles.data(), lift_tiles.size())) ||
-      (rc = upload(&p->dmat_t, dt.data(), dt.size())) ||
+  if (!tri) {
+    std::vector<double> op_tiles, lift_tiles;
+    build_tiles(np, nfp, dsrc, d->lift, op_tiles, lift_tiles);
+    if ((rc = upload(&p->op_tiles, op_tiles.data(), op_tiles.size())) ||
+        (rc = upload(&p->lift_tiles, lift_tiles.data(), lift_tiles.size()))) {
+      free_plan(p);
+      return rc;
+    }
+  }
+  if ((rc = upload(&p->dmat_t, dt.data(), dt.size())) ||
       (rc = upload(&p->lift_t, lt.data(), lt.size())) ||
       (rc = upload(&p->fmask, d->face_nodes, (size_t)nfn)) ||
-      (rc = upload(&p->geo, d->geometry, (size_t)k * kGeo)) ||
+      (rc = upload(&p->geo, d->geometry, (size_t)k * geo_stride)) ||
       (rc = upload(&p->emap, d->ext_map, (size_t)k * nfn))) {
     free_plan(p);
     return rc;
@@ -794,8 +874,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     return fail(DG_ERR_HIP, "hipGetDeviceProperties failed");
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)p->kernel.fn,
-                                                    p->kernel.threads, 0) !=
+  const void* main_fn = tri ? (const void*)p->tri_kernel : (const void*)p->kernel.fn;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, main_fn, p->kernel.threads, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -806,7 +886,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   grid = (grid + 7) / 8 * 8;
   p->grid = (int)grid;
   p->lserk_grid = prop.multiProcessorCount * 8;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 2 && !tri; ++k) {
     const int prec = k == 0 ? DG_PREC_FP32 : DG_PREC_BF16X3;
     std::vector<uint32_t> ow, lw;
     build_mixed_tiles(prec, np, nfp, dsrc, d->lift, ow, lw);
@@ -894,6 +974,17 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
       (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
     return rc;
+  if (p->dim == 2) {
+    Rhs2DArgs<double> a{};
+    for (int c = 0; c < p->nfields; ++c) {
+      a.q[c] = in[c];
+      a.rhs[c] = out[c];
+    }
+    fill_2d_args(p, a);
+    hipLaunchKernelGGL(p->tri_kernel, dim3(p->grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+    DG_HIP_CHECK(hipGetLastError());
+    return DG_OK;
+  }
   RhsArgs a{};
   for (int c = 0; c < p->nfields; ++c) {
     a.q.p[c] = in[c];
@@ -950,6 +1041,8 @@ template <typename T>
 int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T* const* q_bar,
                  T* const* workspace, void* stream, const char* name) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (p->dim == 2)
+    return fail(DG_ERR_UNSUPPORTED, std::string(name) + ": adjoint: tetrahedral plans only");
   if (!p->has_vjp)
     return fail(DG_ERR_UNSUPPORTED, std::string(name) +
                                         ": plan created without mass_inv / face_mass: no adjoint");

@@ -19,16 +19,18 @@ from actx_dgfem_suite_amd.equations._common import box_discretization, make_rhs
 
 def acoustic_pulse_condition(nodes: np.ndarray, gamma: float = 1.4):
     """equations/euler.py:53-137: rho = 1, u = 0, E = p0/(gamma-1) plus a
-    Gaussian pulse exp(-r^2 / (2 w^2)) with w = 0.1 at the origin."""
-    r2 = sum(nodes[i] ** 2 for i in range(3))
+    Gaussian pulse exp(-r^2 / (2 w^2)) with w = 0.1 at the origin, in the
+    nodes' dimension."""
+    dim = nodes.shape[0]
+    r2 = sum(nodes[i] ** 2 for i in range(dim))
     mass = np.ones_like(r2)
     energy = 1.0 / (gamma - 1.0) + np.exp(-0.5 * r2 / 0.1 ** 2)
-    mom = [np.zeros_like(r2) for _ in range(3)]
+    mom = [np.zeros_like(r2) for _ in range(dim)]
     return mass, energy, mom
 
 
 def get_euler_rhs(*, actx, dim: int, order: int, ndofs: int | None = None,
-                  nel_1d: int | None = None, a=(-0.5,) * 3, b=(0.5,) * 3,
+                  nel_1d: int | None = None, a=-0.5, b=0.5,
                   initial_condition=None, gamma: float = 1.4):
     """Returns ``(rhs, (t, fields))`` with ``rhs(t, q) -> q_t``; ``t`` is
     unused, as in the reference (``np.float64(0.5)``, euler.py:186)."""

@@ -37,14 +37,30 @@ def rectangular_cavity_mode(nodes: np.ndarray, t: float = 0.0, mode=(1, 2, 2)):
     return [np.ascontiguousarray(np.real(f)) for f in fields]
 
 
+def rectangular_cavity_mode_2d(nodes: np.ndarray, t: float = 0.0, mode=(2, 3)):
+    """TM mode of the unit square, fields (E_z, H_x, H_y): E_z = sin(kx x)
+    sin(ky y) cos(omega t), H_x = -ky sin(kx x) cos(ky y) sin(omega t) / omega,
+    H_y = kx cos(kx x) sin(ky y) sin(omega t) / omega -- an exact solution of
+    the 2D operator's PDE with PEC walls (mode (2, 3) as in
+    equations/maxwell.py:64-65)."""
+    kx, ky = (m * np.pi for m in mode)
+    omega = np.hypot(kx, ky)
+    x, y = nodes
+    return [np.sin(kx * x) * np.sin(ky * y) * np.cos(omega * t),
+            -ky * np.sin(kx * x) * np.cos(ky * y) * np.sin(omega * t) / omega,
+            kx * np.cos(kx * x) * np.sin(ky * y) * np.sin(omega * t) / omega]
+
+
 def get_maxwell_rhs(*, actx, dim: int, order: int, ndofs: int | None = None,
-                    nel_1d: int | None = None, a=(0.0,) * 3, b=(1.0,) * 3,
+                    nel_1d: int | None = None, a=0.0, b=1.0,
                     initial_condition=None, flux_type: float = 0.5):
     """Returns ``(rhs, (t, fields))``; fields are ``[E_x, E_y, E_z, H_x,
-    H_y, H_z]``.  Defaults: unit cube, cavity mode (1, 2, 2)."""
+    H_y, H_z]`` in 3D and ``[E_z, H_x, H_y]`` (TM) in 2D.  Defaults: unit
+    cube with cavity mode (1, 2, 2), unit square with mode (2, 3)."""
     discr = box_discretization(dim, order, ndofs, nel_1d, a, b)
     if initial_condition is None:
-        leaves = rectangular_cavity_mode(discr.nodes)
+        leaves = (rectangular_cavity_mode(discr.nodes) if dim == 3
+                  else rectangular_cavity_mode_2d(discr.nodes))
     else:
         leaves = list(initial_condition(discr.nodes))
     fields = actx.from_numpy(obj_array(leaves))

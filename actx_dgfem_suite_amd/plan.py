@@ -17,6 +17,9 @@ from actx_dgfem_suite_amd import _lib
 from actx_dgfem_suite_amd.discretization import Discretization
 
 NFIELDS = {"wave": 4, "euler": 5, "maxwell": 6}
+# triangles: wave (u, v_x, v_y), Euler (mass, energy, momentum[2]),
+# Maxwell in TM form (E_z, H_x, H_y)
+NFIELDS_2D = {"wave": 3, "euler": 4, "maxwell": 3}
 
 
 def _dptr(a: np.ndarray, ctype):
@@ -30,6 +33,8 @@ class DGPlan:
     (Maxwell), matching equations/wave.py:279, equations/euler.py:151 and
     equations/maxwell.py:252-254.  ``variant`` picks the FP64 kernel:
     ``"auto"``/``"mfma"`` (matrix cores) or ``"lds"`` (the baseline).
+    Triangle discretizations (``discr.dim == 2``) run the LDS-staged 2D kernel
+    in FP64; their plans have no mixed-precision or adjoint form.
     """
 
     def __init__(self, discr: Discretization, equation: str, param: float,
@@ -45,10 +50,16 @@ class DGPlan:
         self.discr = discr
         self.equation = equation
         self.param = float(param)
-        self.nfields = NFIELDS[equation]
+        self.dim = int(getattr(discr, "dim", 3))
+        if self.dim not in (2, 3):
+            raise NotImplementedError(f"dim={self.dim}")
+        self.nfields = (NFIELDS if self.dim == 3 else NFIELDS_2D)[equation]
         self.device = torch.device("cuda", torch.cuda.current_device()
                                    if device is None else torch.device(device).index or 0)
         ref = discr.ref
+        centroids = discr.nodes.mean(axis=2).T
+        if centroids.shape[1] == 2:
+            centroids = np.concatenate([centroids, np.zeros((centroids.shape[0], 1))], axis=1)
         self._host = {
             "d_weak": np.ascontiguousarray(ref.d_weak, dtype=np.float64),
             "d_strong": np.ascontiguousarray(ref.d_strong, dtype=np.float64),
@@ -59,7 +70,7 @@ class DGPlan:
             "mass_inv": np.ascontiguousarray(ref.mass_inv, dtype=np.float64),
             "face_mass": np.ascontiguousarray(ref.face_mass, dtype=np.float64),
             # element centroids: the plan's locality schedule
-            "centroids": np.ascontiguousarray(discr.nodes.mean(axis=2).T, dtype=np.float64),
+            "centroids": np.ascontiguousarray(centroids, dtype=np.float64),
         }
         h = self._host
         desc = _lib.DgPlanDesc()
@@ -72,8 +83,11 @@ class DGPlan:
         desc.face_nodes = _dptr(h["face_nodes"], ctypes.c_int32)
         desc.geometry = _dptr(h["geometry"], ctypes.c_double)
         desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
-        desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
-        desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
+        if self.dim == 3:
+            # adjoint data (tetrahedra only)
+            desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
+            desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
+        desc.dim = self.dim
         desc.centroids = _dptr(h["centroids"], ctypes.c_double)
         desc.params[0] = self.param
         desc.variant = _lib.KERNEL_VARIANTS[variant]

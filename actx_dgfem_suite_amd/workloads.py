@@ -96,10 +96,13 @@ def maxwell_target(discr: Discretization, seed: int = 1) -> np.ndarray:
 
 # {{{ cost models (per RHS evaluation)
 
-def algorithmic_bytes(k: int, np_: int, nfp: int, nfields: int, state_bytes: int = 8) -> int:
-    """HBM bytes of one RHS: state read once + RHS written once, 25 doubles of
-    affine geometry per element, int32 exterior-node map."""
-    return k * (2 * nfields * np_ * state_bytes + 25 * 8) + 4 * k * 4 * nfp
+def algorithmic_bytes(k: int, np_: int, nfp: int, nfields: int, state_bytes: int = 8,
+                      dim: int = 3) -> int:
+    """HBM bytes of one RHS: state read once + RHS written once, the affine
+    geometry per element (25 doubles for a tetrahedron, 13 for a triangle),
+    int32 exterior-node map (dim + 1 faces of Nfp nodes)."""
+    geo = 25 if dim == 3 else 13
+    return k * (2 * nfields * np_ * state_bytes + geo * 8) + 4 * k * (dim + 1) * nfp
 
 
 _VOLUME_MATVECS = {"wave": 6, "euler": 15, "maxwell": 18}

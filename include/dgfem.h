@@ -16,8 +16,10 @@
  *    dg_last_error_string() gives a message for the calling thread.
  *  - Field data are device pointers to C-contiguous (K, Np) float64 arrays, one
  *    per field, element-major: the layout of one meshmode DOFArray group.
- *    Field order: wave (u, v_x, v_y, v_z); Euler (mass, energy, momentum
- *    x, y, z); Maxwell (E_x, E_y, E_z, H_x, H_y, H_z).
+ *    Field order, tetrahedra: wave (u, v_x, v_y, v_z); Euler (mass, energy,
+ *    momentum x, y, z); Maxwell (E_x, E_y, E_z, H_x, H_y, H_z).  Triangles
+ *    (dim = 2): wave (u, v_x, v_y); Euler (mass, energy, momentum x, y);
+ *    Maxwell in TM form (E_z, H_x, H_y).
  *  - `fields_in` / `fields_out` are HOST arrays of nfields DEVICE pointers.
  *  - Launches are asynchronous on `stream`; nothing synchronises the host.
  *  - Hot-path calls allocate nothing.  Plans own their device constants.
@@ -57,33 +59,37 @@ typedef enum dg_precision {
   DG_PREC_BF16X3 = 2  /* mixed: as FP32 with D/LIFT as a 3-product bf16 split */
 } dg_precision;
 
-/* Everything a plan needs; all pointers are HOST memory and are copied. */
+/* Everything a plan needs; all pointers are HOST memory and are copied.
+ * Shapes are given for tetrahedra (d = 3 directions, F = 4 faces); triangles
+ * (dim = 2) have d = 2, F = 3, Np = (N+1)(N+2)/2 and Nfp = N+1. */
 typedef struct dg_plan_desc {
   int32_t equation;          /* dg_equation */
   int32_t order;             /* polynomial degree, 1..4 */
   int64_t nelements;         /* K */
-  /* (3, Np, Np) row-major [r][i][j]: M^{-1} S_r^T, the weak derivative with
+  /* (d, Np, Np) row-major [r][i][j]: M^{-1} S_r^T, the weak derivative with
    * the inverse mass fused (mass_inverse_fuser.py:39-74) */
   const double* d_weak;
-  /* (3, Np, Np): V_r V^{-1}, the strong reference derivative */
+  /* (d, Np, Np): V_r V^{-1}, the strong reference derivative */
   const double* d_strong;
-  /* (Np, 4*Nfp): M^{-1} E, face mass fused with inverse mass
+  /* (Np, F*Nfp): M^{-1} E, face mass fused with inverse mass
    * (mass_inverse_fuser.py:21-38) */
   const double* lift;
-  /* (4, Nfp): volume node index of each face node */
+  /* (F, Nfp): volume node index of each face node */
   const int32_t* face_nodes;
-  /* (K, 25): rx[x][r] = dr_r/dx_x (9 values), then for faces f=0..3:
-   * n_x, n_y, n_z, Fscale = sJ/J */
+  /* Tetrahedra (K, 25): rx[x][r] = dr_r/dx_x (9 values), then for faces
+   * f=0..3: n_x, n_y, n_z, Fscale = sJ/J.  Triangles (K, 13): rx[x][r] (4
+   * values), then for edges f=0..2: n_x, n_y, Fscale. */
   const double* geometry;
-  /* (K, 4*Nfp): flat index e2*Np + i2 of the exterior node of every face
+  /* (K, F*Nfp): flat index e2*Np + i2 of the exterior node of every face
    * node; boundary face nodes hold -(e*Np + i) - 1 (their own node, negated) */
   const int32_t* ext_map;
   /* wave: params[0] = c; Euler: params[0] = gamma; Maxwell: params[0] =
    * flux_type alpha (0 central .. 1 upwind), with epsilon = mu = 1 as in
    * equations/maxwell.py:252-254 */
   double params[4];
-  int32_t variant;           /* dg_kernel_variant */
-  int32_t reserved;          /* must be 0 */
+  int32_t variant;           /* dg_kernel_variant (triangles: AUTO or LDS) */
+  int32_t dim;               /* 3 (or 0): tetrahedra; 2: triangles, which run
+                              * the LDS-staged kernel, FP64 only, no adjoint */
   /* Needed by dg_rhs_vjp only (both NULL for a forward-only plan):
    * (Np, Np) reference inverse mass M^{-1}, and (Np, 4*Nfp) reference face
    * mass E with LIFT = M^{-1} E (mass_inverse_fuser.py:21-38 is the fusion
@@ -128,8 +134,10 @@ int dg_rhs(const dg_plan* plan, int32_t precision,
  * contractions run on the matrix cores in FP32 (or a bf16x3 split) with FP32
  * accumulation; geometric factors, the numerical flux (Euler pressure
  * included) and the combination of reference into physical derivatives run
- * in FP64.  It has no counterpart in the reference, whose array contexts
- * evaluate the operator in the state's own precision. */
+ * in FP64.  Triangle plans (dim = 2) take FP32 state and RHS under both
+ * precisions and compute everything in FP64.  It has no counterpart in the
+ * reference, whose array contexts evaluate the operator in the state's own
+ * precision. */
 int dg_rhs_mixed(const dg_plan* plan, int32_t precision,
                  const float* const* fields_in, float* const* fields_out,
                  void* stream);

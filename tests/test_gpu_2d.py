@@ -1,0 +1,153 @@
+"""GPU parity on triangles (dim = 2): the 2D HIP kernel against the oracle's
+2D operators, its FP32-storage form, the LSERK4 update, the 2D builders
+behind the array context, and the runner's ``--dims 2,3`` matrix (the
+reference's CI job, .github/workflows/ci.yml).  Calls go through the C ABI."""
+
+import numpy as np
+import pytest
+import torch
+
+from actx_dgfem_suite_amd.containers import flatten_fields
+from actx_dgfem_suite_amd.plan import DGPlan
+from actx_dgfem_suite_amd.rhs_builder import get_rhs
+from oracle import dg_oracle
+from tests._helpers import discr_pair_2d, field_errors, oracle_rhs, random_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+PARAM = {"wave": 1.0, "euler": 1.4, "maxwell": 0.5}
+TOL_FP64 = 1e-12
+
+
+def _run(plan, q_np, device, dtype=torch.float64, precision="fp64"):
+    q = [torch.from_numpy(np.ascontiguousarray(f)).to(device=device, dtype=dtype) for f in q_np]
+    out = plan.rhs(q, precision=precision)
+    torch.cuda.synchronize()
+    return np.stack([t.cpu().numpy().astype(np.float64) for t in out])
+
+
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_rhs_2d_matches_oracle(equation, order, device):
+    print(f"[case] 2d rhs {equation} P{order} n=5", flush=True)
+    d, od = discr_pair_2d(order, 5)
+    q = random_state(equation, od.x, seed=order)
+    plan = DGPlan(d, equation, PARAM[equation])
+    assert plan.kernel_variant == "lds" and plan.nfields == q.shape[0]
+    got = _run(plan, q, device)
+    errs = field_errors(got, oracle_rhs(equation, od, q, PARAM[equation]))
+    print(f"  rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < TOL_FP64, errs
+
+
+@pytest.mark.parametrize("equation,order,nel", [("wave", 1, 340), ("euler", 4, 170),
+                                                ("maxwell", 3, 200)])
+def test_rhs_2d_matches_oracle_many_batches(equation, order, nel, device):
+    """Meshes large enough that every persistent block runs several batches
+    (231200, 57800 and 80000 triangles)."""
+    print(f"[case] 2d rhs {equation} P{order} n={nel}", flush=True)
+    d, od = discr_pair_2d(order, nel)
+    q = random_state(equation, od.x, seed=11)
+    plan = DGPlan(d, equation, PARAM[equation])
+    got = _run(plan, q, device)
+    errs = field_errors(got, oracle_rhs(equation, od, q, PARAM[equation]))
+    assert max(errs) < TOL_FP64, errs
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+def test_rhs_2d_fp32_storage(equation, precision, device):
+    """Triangle plans' mixed form: FP32 fields in and out, FP64 arithmetic.
+    Against the oracle on the FP32-rounded state, only the output's rounding
+    remains (relative 2^-24 per value)."""
+    print(f"[case] 2d rhs {precision} {equation} P3", flush=True)
+    d, od = discr_pair_2d(3, 6)
+    q = random_state(equation, od.x, seed=5).astype(np.float32)
+    plan = DGPlan(d, equation, PARAM[equation])
+    got = _run(plan, q, device, torch.float32, precision)
+    ref = oracle_rhs(equation, od, q.astype(np.float64), PARAM[equation])
+    errs = field_errors(got, ref)
+    assert max(errs) < 2e-7, errs
+
+
+def test_lserk_2d_step_matches_oracle(device):
+    print("[case] 2d lserk euler P2", flush=True)
+    d, od = discr_pair_2d(2, 6)
+    q = random_state("euler", od.x, seed=4)
+    plan = DGPlan(d, "euler", 1.4)
+    dt = 1e-3
+    qt = [torch.from_numpy(f.copy()).to(device) for f in q]
+    res = [torch.zeros_like(t) for t in qt]
+    rhs = plan.empty_fields()
+    for a, b in zip(dg_oracle.LSRK4_A, dg_oracle.LSRK4_B):
+        plan.rhs(qt, rhs)
+        plan.lserk_stage(a, b, dt, qt, res, rhs)
+    torch.cuda.synchronize()
+    got = np.stack([t.cpu().numpy() for t in qt])
+    ref = dg_oracle.lsrk4_step(lambda t, s: oracle_rhs("euler", od, s, 1.4), q, dt)
+    assert rel_l2(got, ref) < 1e-13
+
+
+def test_2d_plans_have_no_mfma_or_adjoint(device):
+    d, _ = discr_pair_2d(1, 2)
+    with pytest.raises(NotImplementedError, match="MFMA"):
+        DGPlan(d, "wave", 1.0, variant="mfma")
+    plan = DGPlan(d, "wave", 1.0)
+    y = plan.empty_fields()
+    with pytest.raises(NotImplementedError, match="adjoint"):
+        plan.rhs_vjp(None, y)
+
+
+@pytest.mark.parametrize("equation", ["tiny_wave", "tiny_euler", "tiny_maxwell"])
+@pytest.mark.parametrize("degree", [1, 4])
+def test_2d_builders_through_the_array_context(equation, degree, device):
+    """get_{wave,euler,maxwell}_rhs(dim=2) compiled by the array context:
+    unit-square / [-0.5, 0.5]^2 meshes, the reference's 2D initial states."""
+    from actx_dgfem_suite_amd.arraycontext import HIPDGFEMArrayContext
+    print(f"[case] 2d actx {equation} P{degree}", flush=True)
+    actx = HIPDGFEMArrayContext()
+    rhs, args = get_rhs(equation, actx, 2, degree)
+    out = actx.compile(rhs)(*args)
+    actx.queue.finish()
+    op = rhs.dg_operator
+    d = op.discr
+    assert d.dim == 2
+    od = dg_oracle.build_discretization_2d(d.mesh.vertices, d.mesh.elements, d.ref.nodes)
+    q = np.stack(flatten_fields(actx.to_numpy(args[-1])))
+    got = np.stack(flatten_fields(actx.to_numpy(out)))
+    assert max(field_errors(got, oracle_rhs(op.equation, od, q, op.param))) < 1e-12
+
+
+def test_2d_maxwell_cavity_mode_time_derivative(device):
+    """The TM cavity mode (2, 3) solves the PDE exactly: at t = 0 its time
+    derivative is (0, -ky sin(kx x) cos(ky y), kx cos(kx x) sin(ky y)).  The
+    P4 DG operator reproduces it with 4th-order convergence under h -> h/2."""
+    from actx_dgfem_suite_amd.arraycontext import HIPDGFEMArrayContext
+    from actx_dgfem_suite_amd.equations.maxwell import get_maxwell_rhs
+    actx = HIPDGFEMArrayContext()
+    errs = []
+    for nel in (8, 16):
+        rhs, (t, fields) = get_maxwell_rhs(actx=actx, dim=2, order=4, nel_1d=nel)
+        got = np.stack(flatten_fields(actx.to_numpy(actx.compile(rhs)(t, fields))))
+        x, y = rhs.dg_operator.discr.nodes
+        kx, ky = 2 * np.pi, 3 * np.pi
+        want = np.stack([np.zeros_like(x), -ky * np.sin(kx * x) * np.cos(ky * y),
+                         kx * np.cos(kx * x) * np.sin(ky * y)])
+        errs.append(np.linalg.norm(got - want) / np.linalg.norm(want))
+    print(f"  cavity mode d/dt rel L2 = {errs}", flush=True)
+    assert errs[1] < 5e-4
+    assert 12 < errs[0] / errs[1] < 20
+
+
+def test_runner_ci_matrix(device, capsys):
+    """The reference's CI invocation: --dims 2,3 --degrees 1,2,3,4 on the tiny
+    tiers, with the value check on."""
+    from actx_dgfem_suite_amd.run.__main__ import _NAME_TO_ACTX_CLASS, main
+    print("[case] runner CI matrix", flush=True)
+    rates = main(equations=["tiny_euler", "tiny_maxwell", "tiny_wave"], dims=[2, 3],
+                 degrees=[1, 2, 3, 4], actx_ts=[_NAME_TO_ACTX_CLASS["hip:dgfem"]], verify=True)
+    assert rates.shape == (1, 2, 3, 4) and np.all(rates > 0)
+    out = capsys.readouterr().out
+    for dim in (2, 3):
+        for eq in ("tiny_euler", "tiny_maxwell", "tiny_wave"):
+            assert f"GFLOPS/s for {dim}D-{eq}" in out

@@ -8,12 +8,14 @@ import pytest
 
 from actx_dgfem_suite_amd.consts import lookup_device
 from actx_dgfem_suite_amd.perf_analysis import (
+    box_mesh_counts,
     einsum_flops,
     get_float64_flops,
     get_footprint_bytes,
     get_roofline_flop_rate,
     mesh_counts,
 )
+from actx_dgfem_suite_amd.utils import get_nel_1d_for_regular_rect_mesh
 from actx_dgfem_suite_amd.workloads import algorithmic_bytes
 
 
@@ -85,7 +87,7 @@ def test_unknown_names_raise():
     with pytest.raises(NotImplementedError):
         get_float64_flops("burgers", 3, 1)
     with pytest.raises(NotImplementedError):
-        mesh_counts("wave", 2, 1)
+        mesh_counts("wave", 1, 1)
     from actx_dgfem_suite_amd.measure import finish_command_queue, instantiate_actx_t
     with pytest.raises(NotImplementedError):
         instantiate_actx_t(object)
@@ -107,3 +109,31 @@ def test_dof_scaling_command_line():
     a = _parse(["--dim", "3", "--degrees", "2,3", "--ndofs", "1e6,4e6"])
     assert a.dim == 3 and a.degrees == "2,3" and a.ndofs == "1e6,4e6"
     assert _parse([]).ndofs == ",".join(str(n) for n in DEFAULT_NDOFS)
+
+
+@pytest.mark.parametrize("degree,nel", [(1, 3), (2, 4), (4, 2)])
+def test_triangle_mesh_counts_match_the_mesh(degree, nel):
+    """2D sizes: 2 n^2 triangles, 4 n boundary edges, Np and Nfp of the
+    reference triangle -- read off an actual square mesh."""
+    from actx_dgfem_suite_amd.discretization.triangle import make_square_discretization
+    d = make_square_discretization(degree, nel)
+    m = box_mesh_counts(degree, nel, dim=2)
+    assert (m.k, m.np, m.nfp, m.nfaces) == (d.nelements, d.np, d.nfp, 3)
+    assert m.nbdry == int((d.ext_map[:, ::d.nfp] < 0).sum())
+    assert m.nall == d.ext_map.size
+    assert m.nbnd == int((d.ext_map < 0).sum())
+
+
+def test_triangle_footprint_and_flops():
+    """2D footprint: 3 or 4 fields in and out, 13 geometry doubles and 3 Nfp
+    map entries per triangle; the counting rule's contractions in 2D are the
+    einsums with x, r of size 2 and 3 faces."""
+    m = box_mesh_counts(3, 5, dim=2)
+    assert get_footprint_bytes("tiny_wave", 2, 3) == \
+        box_mesh_counts(3, get_nel_1d_for_regular_rect_mesh(2, 3, 1000), dim=2).k * \
+        (2 * 3 * 10 * 8 + 13 * 8 + 4 * 3 * 4)
+    lift = einsum_flops("ifj,fe,fej->ei", {"i": 10, "f": 3, "j": 4, "e": m.k})
+    assert lift == m.k * 3 * 4 * 10 * 2 + m.k * 3 * 4       # scale, then matvec
+    for eq in ("tiny_wave", "tiny_euler", "tiny_maxwell"):
+        f = [get_float64_flops(eq, 2, deg) for deg in (1, 2, 3, 4)]
+        assert all(x > 0 for x in f)
