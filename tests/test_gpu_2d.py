@@ -44,13 +44,27 @@ def test_rhs_2d_matches_oracle(equation, order, device):
                                                 ("maxwell", 3, 200)])
 def test_rhs_2d_matches_oracle_many_batches(equation, order, nel, device):
     """Meshes large enough that every persistent block runs several batches
-    (231200, 57800 and 80000 triangles)."""
+    (231200, 57800 and 80000 triangles).  On such fine meshes the RHS is a
+    small difference of large volume and face terms (Euler P4 momentum at
+    h = 1/170: 34x), and FP64 rounding is relative to those terms, so each
+    field's error is measured against the larger of its RHS and its
+    inverse-mass-scaled volume term."""
     print(f"[case] 2d rhs {equation} P{order} n={nel}", flush=True)
     d, od = discr_pair_2d(order, nel)
     q = random_state(equation, od.x, seed=11)
     plan = DGPlan(d, equation, PARAM[equation])
     got = _run(plan, q, device)
-    errs = field_errors(got, oracle_rhs(equation, od, q, PARAM[equation]))
+    ref = oracle_rhs(equation, od, q, PARAM[equation])
+    if equation == "euler":
+        flux = dg_oracle.euler_volume_flux(q, PARAM[equation])
+        vol = [dg_oracle.inverse_mass(od, sum(dg_oracle.weak_local_d_dx(od, x, flux[c][x])
+                                              for x in range(2)))
+               for c in range(len(q))]
+    else:
+        vol = ref
+    errs = [np.linalg.norm(got[c] - ref[c]) / max(np.linalg.norm(ref[c]), np.linalg.norm(vol[c]))
+            for c in range(len(q))]
+    print(f"  rel L2 per field (term scale): {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) < TOL_FP64, errs
 
 
