@@ -46,6 +46,11 @@ struct RhsArgs {
   const int32_t* fmask;   // (4*Nfp)
   const double* op_tiles;    // MFMA A tiles of D: [r][mblock][kstep][64]
   const double* lift_tiles;  // MFMA A tiles of LIFT: [mblock][kstep][64]
+  // MFMA kernel, tile-ordered copies of geometry and exterior map (16-element
+  // tiles, the last one padded with its final element):
+  const double* vgeo;        // [tile][9][16]: rx of element n of the tile
+  const double* fgeo;        // [tile][face*16 + n][4]: n_x, n_y, n_z, Fscale
+  const int32_t* ext_t;      // [tile][chunk][item][64]: emap in flux-item order
   int64_t nelem;
   double p0;
 };

@@ -10,12 +10,15 @@
 // Per tile:
 //   volume: B operands straight from HBM into registers (lane group g loads a
 //     contiguous run of each element's nodes), A operands (pre-tiled
-//     operators) from LDS;
+//     operators) from LDS, rx from the tile-ordered geometry (one cache line
+//     per load);
 //   surface, C LIFT k-steps at a time: the wave evaluates the numerical flux
 //     in a gather-friendly order (consecutive lanes = consecutive nodes of one
 //     face, so the exterior gathers of neighbouring lanes hit the same
-//     neighbour element), hands it to itself through a small LDS buffer in
-//     MFMA B order and runs those LIFT k-steps;
+//     neighbour element), with normals, Fscale and exterior-map entries read
+//     from per-tile tables laid out in that item order (contiguous loads),
+//     hands it to itself through a small LDS buffer in MFMA B order and runs
+//     those LIFT k-steps;
 //   output: accumulators go through the same buffer to contiguous stores.
 //
 // Operand layouts of v_mfma_f64_16x16x4f64 (lane l, g = l / 16, n = l % 16),
@@ -68,11 +71,21 @@ constexpr bool prefetch_ext() {
   return !(EQ == DG_EQ_WAVE && (ORDER == 1 || ORDER == 3));
 }
 
-// LIFT k-steps whose flux one wave stages in LDS at a time.
+// LIFT k-steps whose flux one wave stages in LDS at a time (P1, P2: one
+// chunk; P3, P4: 5 nodes).  Host code sizes the flux-item tables with it.
+__host__ __device__ constexpr int flux_chunk_of(int order) {
+  return nfp_of(order) <= 6 ? nfp_of(order) : 5;
+}
 template <int EQ, int ORDER>
-constexpr int flux_chunk() {
-  constexpr int nfp = nfp_of(ORDER);
-  return nfp <= 6 ? nfp : 5;                 // P1, P2: one chunk; P3, P4: 5 nodes
+constexpr int flux_chunk() { return flux_chunk_of(ORDER); }
+
+// Flux items of one tile: chunk ch, slot it, lane l handle item it*64 + l of
+// that chunk, i.e. face-pair p = item / C (p = face * 16 + element) at LIFT
+// k-step ch * C + item % C.  ext_t holds the exterior-map entry of every
+// item in this order, so each slot's entries are one contiguous 256-byte load.
+__host__ __device__ constexpr int flux_items_of(int order) {
+  return ((nfp_of(order) + flux_chunk_of(order) - 1) / flux_chunk_of(order)) *
+         flux_chunk_of(order) * 64;
 }
 
 constexpr int kFluxRow = 65;                 // padded flux row (spreads LDS banks)
@@ -145,15 +158,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   int32_t ext_pf[C];
   auto load_ext = [&](int64_t t, int ch) {
 #pragma unroll
-    for (int it = 0; it < C; ++it) {
-      const int item = it * 64 + lane;
-      const int kk = item % C;
-      const int p = item / C;
-      const int ks = ch * C + kk;
-      int64_t pe = t * 16 + (p & 15);
-      pe = pe < a.nelem ? pe : a.nelem - 1;
-      ext_pf[it] = ks < NFP ? a.emap[pe * NFN + (p >> 4) * NFP + ks] : 0;
-    }
+    for (int it = 0; it < C; ++it)
+      ext_pf[it] = a.ext_t[t * flux_items_of(ORDER) + (ch * C + it) * 64 + lane];
   };
 
   for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
@@ -162,10 +168,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
-    const double* geo = a.geo + el * kGeo;
     double rx[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rx[k] = geo[k];
+    for (int k = 0; k < 9; ++k) rx[k] = a.vgeo[(tile * 9 + k) * 16 + n];
 
     f64x4 acc[NF][MB];
 #pragma unroll
@@ -309,15 +314,17 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           const int fg = p >> 4, fe = p & 15;
           int64_t pe = tile * 16 + fe;
           pe = pe < a.nelem ? pe : a.nelem - 1;
-          const double* pgeo = a.geo + pe * kGeo + 9 + 4 * fg;
-          const double nrm[3] = {pgeo[0], pgeo[1], pgeo[2]};
-          const double fscale = pgeo[3];
+          const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + p) * 4);
+          const double2 g01 = pgeo[0], g23 = pgeo[1];
+          const double nrm[3] = {g01.x, g01.y, g23.x};
+          const double fscale = g23.y;
           const int fn = fg * NFP + ks;
           const int64_t vn = pe * NP + s_fmask[fn];
           double qm[NF], qp[NF], out[NF];
 #pragma unroll
           for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
-          const int32_t ext = PF ? ext_pf[it] : a.emap[pe * NFN + fn];
+          const int32_t ext =
+              PF ? ext_pf[it] : a.ext_t[tile * flux_items_of(ORDER) + (ch * C + it) * 64 + lane];
           if (ext >= 0) {
 #pragma unroll
             for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
