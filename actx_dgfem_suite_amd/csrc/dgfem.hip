@@ -529,20 +529,25 @@ __global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
 void build_tiles(int np, int nfp, const double* op, const double* lift,
                  std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
   const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, nfn = kFaces * nfp;
+  // A row of lane l in block b: l % 16 for v_mfma_f64_16x16x4f64, l % 4 for
+  // the 4x4x4 form a short last block runs on
+  auto row = [&](int b, int l) {
+    return 16 * b + ((b == mb - 1 && short_last_block(np)) ? (l & 3) : (l & 15));
+  };
   op_tiles.assign((size_t)3 * mb * ks_n * 64, 0.0);
   lift_tiles.assign((size_t)mb * nfp * 64, 0.0);
   for (int r = 0; r < 3; ++r)
     for (int b = 0; b < mb; ++b)
       for (int ks = 0; ks < ks_n; ++ks)
         for (int l = 0; l < 64; ++l) {
-          const int m = 16 * b + (l & 15), j = (l >> 4) * ks_n + ks;
+          const int m = row(b, l), j = (l >> 4) * ks_n + ks;
           if (m < np && j < np)
             op_tiles[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] = op[((size_t)r * np + m) * np + j];
         }
   for (int b = 0; b < mb; ++b)
     for (int ks = 0; ks < nfp; ++ks)
       for (int l = 0; l < 64; ++l) {
-        const int m = 16 * b + (l & 15), fn = (l >> 4) * nfp + ks;
+        const int m = row(b, l), fn = (l >> 4) * nfp + ks;
         if (m < np) lift_tiles[((size_t)b * nfp + ks) * 64 + l] = lift[(size_t)m * nfn + fn];
       }
 }

@@ -50,6 +50,30 @@ __device__ inline f64x4 mfma_f64(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Output blocks of at most 4 rows (the last block of P1, P3 and P4: Np = 4,
+// 20 = 16 + 4, 35 = 32 + 3) run on v_mfma_f64_4x4x4f64, which takes 0.28x the
+// time of the 16x16x4 form on gfx950.  Its four 4x4 blocks tile the 16
+// columns, and, measured on gfx950, its B operand layout is the 16x16x4 one
+// (lane l: B[k = l / 16][col = l % 16]) and its result is the 16x16x4
+// result's register v = 0 (lane l: C[row = l / 16][col = l % 16]); only A
+// differs (lane l: A[row = l % 4][k = l / 16]), which the host's A tiles
+// follow.  So a short block keeps the same B operands and uses component 0 of
+// its accumulator.
+__host__ __device__ constexpr bool short_last_block(int np) {
+  return np - 16 * ((np + 15) / 16 - 1) <= 4;
+}
+
+// acc += A B on output block b (short: component 0 only).
+__device__ inline void mma(f64x4& acc, double a, double b, bool short_block) {
+  if (short_block)
+    acc[0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[0], 0, 0, 0);
+  else
+    acc = mfma_f64(a, b, acc);
+}
+
+// Components of an accumulator that hold rows: 1 for a short block, else 4.
+__device__ constexpr int ncomp(bool short_block) { return short_block ? 1 : 4; }
+
 template <int ORDER>
 struct MfmaShape {
   static constexpr int NP = np_of(ORDER);
@@ -122,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   constexpr int NF = nfields_of(EQ);
   constexpr int WAVES = kBlock / 64;
 
+  constexpr bool SH = short_last_block(NP);   // last output block on 4x4x4
   constexpr int C = flux_chunk<EQ, ORDER>();
   constexpr int NCHUNK = (NFP + C - 1) / C;
   constexpr int WB = wave_buffer<EQ, ORDER>();
@@ -204,19 +229,21 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
 #pragma unroll
           for (int b = 0; b < MB; ++b) {
             const double av = s_op[((r * MB + b) * KS + ks) * 64 + lane];
-            t[r][b] = mfma_f64(av, u, t[r][b]);
-            acc[0][b] = mfma_f64(av, w, acc[0][b]);
+            mma(t[r][b], av, u, SH && b == MB - 1);
+            mma(acc[0][b], av, w, SH && b == MB - 1);
           }
         }
       }
 #pragma unroll
-      for (int b = 0; b < MB; ++b) {
-        acc[0][b] *= -c0;
+      for (int b = 0; b < MB; ++b)
 #pragma unroll
-        for (int x = 0; x < 3; ++x)
-          acc[1 + x][b] = -c0 * (rx[3 * x] * t[0][b] + rx[3 * x + 1] * t[1][b] +
-                                 rx[3 * x + 2] * t[2][b]);
-      }
+        for (int v = 0; v < ncomp(SH && b == MB - 1); ++v) {
+          acc[0][b][v] *= -c0;
+#pragma unroll
+          for (int x = 0; x < 3; ++x)
+            acc[1 + x][b][v] = -c0 * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
+                                      rx[3 * x + 2] * t[2][b][v]);
+        }
     } else if constexpr (EQ == DG_EQ_EULER) {
       // q_t = sum_r D_r w_{c,r},  w_{c,r} = sum_x rx[x][r] F_{c,x}(q)
       const double gamma = c0;
@@ -255,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           for (int b = 0; b < MB; ++b) {
             const double av = s_op[((r * MB + b) * KS + ks) * 64 + lane];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) acc[c][b] = mfma_f64(av, w[c][r], acc[c][b]);
+            for (int c = 0; c < 5; ++c) mma(acc[c][b], av, w[c][r], SH && b == MB - 1);
           }
       }
     } else {
@@ -275,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int b = 0; b < MB; ++b)
-              t[r][b] = mfma_f64(s_op[((r * MB + b) * KS + ks) * 64 + lane], u, t[r][b]);
+              mma(t[r][b], s_op[((r * MB + b) * KS + ks) * 64 + lane], u, SH && b == MB - 1);
         }
         const int comp = c % 3;
         const bool is_h = c >= 3;
@@ -290,8 +317,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
             const int out = is_h ? o : 3 + o;
 #pragma unroll
             for (int b = 0; b < MB; ++b)
-              acc[out][b] += sign * (rx[3 * x] * t[0][b] + rx[3 * x + 1] * t[1][b] +
-                                     rx[3 * x + 2] * t[2][b]);
+#pragma unroll
+              for (int v = 0; v < ncomp(SH && b == MB - 1); ++v)
+                acc[out][b][v] += sign * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
+                                          rx[3 * x + 2] * t[2][b][v]);
           }
         }
       }
@@ -344,12 +373,14 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
       for (int kk = 0; kk < C; ++kk) {
         const int ks = ch * C + kk;
         if (ks < NFP) {
+          double bv[NF];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) bv[c] = buf[(kk * NF + c) * FROW + lane];
 #pragma unroll
           for (int b = 0; b < MB; ++b) {
             const double av = s_lift[(b * NFP + ks) * 64 + lane];
 #pragma unroll
-            for (int c = 0; c < NF; ++c)
-              acc[c][b] = mfma_f64(av, buf[(kk * NF + c) * FROW + lane], acc[c][b]);
+            for (int c = 0; c < NF; ++c) mma(acc[c][b], av, bv[c], SH && b == MB - 1);
           }
         }
       }
