@@ -106,7 +106,8 @@ struct EulerFlux {
   __device__ static void normal_flux(const double* q, const double* n, double gamma,
                                      double* fn, double* lam) {
     const double rho = q[0], e = q[1];
-    const double ux = q[2] / rho, uy = q[3] / rho, uz = q[4] / rho;
+    const double inv_rho = 1.0 / rho;
+    const double ux = q[2] * inv_rho, uy = q[3] * inv_rho, uz = q[4] * inv_rho;
     const double p = (gamma - 1.0) * (e - 0.5 * (q[2] * ux + q[3] * uy + q[4] * uz));
     const double un = ux * n[0] + uy * n[1] + uz * n[2];
     fn[0] = q[2] * n[0] + q[3] * n[1] + q[4] * n[2];
@@ -114,7 +115,7 @@ struct EulerFlux {
     fn[2] = q[2] * un + p * n[0];
     fn[3] = q[3] * un + p * n[1];
     fn[4] = q[4] * un + p * n[2];
-    *lam = sqrt(ux * ux + uy * uy + uz * uz) + sqrt(gamma * (p / rho));
+    *lam = sqrt(ux * ux + uy * uy + uz * uz) + sqrt(gamma * p * inv_rho);
   }
   __device__ static void flux(const double* qm, const double* qp, const double* n,
                               double gamma, double* out) {

@@ -209,13 +209,14 @@ MixedKernel pick_mixed(int prec, int eq, int order) {
   return nullptr;
 }
 
-// The FP64 kernel "auto" picks, from measurements on MI355X at 4M DOFs per
-// field: the MFMA kernel for wave and Euler, the LDS-staged kernel for
-// Maxwell, whose 18 reference-derivative contractions per element make the
-// MFMA kernel register-bound (one wave per SIMD from P2, spills at P4) while
-// the LDS-staged kernel keeps 2-3 waves per SIMD (P1: within 3%).
-int auto_variant(int equation, int /*order*/) {
-  return equation == DG_EQ_MAXWELL ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
+// The FP64 kernel "auto" picks, from measurements on MI355X at 3-4M DOFs
+// per field: the MFMA kernel everywhere except Maxwell P2, where the LDS-staged
+// kernel is 20% faster (Np = 10 fills one 16-row MFMA block to 62%, and the 18
+// reference-derivative contractions per element hold the MFMA kernel at one
+// wave per SIMD).  Elsewhere the MFMA kernel leads by 10-60% (Maxwell P4:
+// 324 vs 360 us, wave P3: 112 vs 173 us).
+int auto_variant(int equation, int order) {
+  return (equation == DG_EQ_MAXWELL && order == 2) ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
 }
 
 // {{{ locality schedule

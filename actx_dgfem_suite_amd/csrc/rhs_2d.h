@@ -59,14 +59,15 @@ struct Euler2DFlux {
   __device__ static void normal_flux(const double* q, const double* n, double gamma,
                                      double* fn, double* lam) {
     const double rho = q[0], e = q[1];
-    const double ux = q[2] / rho, uy = q[3] / rho;
+    const double inv_rho = 1.0 / rho;
+    const double ux = q[2] * inv_rho, uy = q[3] * inv_rho;
     const double p = (gamma - 1.0) * (e - 0.5 * (q[2] * ux + q[3] * uy));
     const double un = ux * n[0] + uy * n[1];
     fn[0] = q[2] * n[0] + q[3] * n[1];
     fn[1] = (e + p) * un;
     fn[2] = q[2] * un + p * n[0];
     fn[3] = q[3] * un + p * n[1];
-    *lam = sqrt(ux * ux + uy * uy) + sqrt(gamma * (p / rho));
+    *lam = sqrt(ux * ux + uy * uy) + sqrt(gamma * p * inv_rho);
   }
   __device__ static void flux(const double* qm, const double* qp, const double* n,
                               double gamma, double* out) {
@@ -201,16 +202,18 @@ __global__ __launch_bounds__(kBlock) void rhs_2d_kernel(Rhs2DArgs<T> a) {
       } else if constexpr (EQ == DG_EQ_EULER) {
         const double gamma = c0;
         const double rho = s_q[0][node], e = s_q[1][node];
+        const double inv_rho = 1.0 / rho;
         const double m[2] = {s_q[2][node], s_q[3][node]};
-        const double u[2] = {m[0] / rho, m[1] / rho};
+        const double u[2] = {m[0] * inv_rho, m[1] * inv_rho};
         const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1]));
+        const double h = (e + p) * inv_rho;
         double f[4][2];
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
           f[0][x] = m[x];
-          f[1][x] = m[x] * (e + p) / rho;
+          f[1][x] = m[x] * h;
 #pragma unroll
-          for (int y = 0; y < 2; ++y) f[2 + y][x] = m[y] * m[x] / rho + (x == y ? p : 0.0);
+          for (int y = 0; y < 2; ++y) f[2 + y][x] = m[y] * u[x] + (x == y ? p : 0.0);
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)

@@ -1,6 +1,6 @@
-// Baseline fused RHS kernel: one thread per (element, node), element batches
-// and the transposed operators staged in LDS.  Kept as the reference variant
-// the MFMA kernel is measured against.
+// Fused RHS kernel with one thread per (element, node), element batches and
+// the transposed operators staged in LDS: the baseline the MFMA kernel is
+// measured against, and the "auto" choice where it is faster (Maxwell P2).
 #pragma once
 
 #include "dg_common.h"
@@ -98,17 +98,19 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       } else if constexpr (EQ == DG_EQ_EULER) {
         const double gamma = c0;
         const double rho = s_q[0][node], e = s_q[1][node];
+        const double inv_rho = 1.0 / rho;
         const double m[3] = {s_q[2][node], s_q[3][node], s_q[4][node]};
-        const double u[3] = {m[0] / rho, m[1] / rho, m[2] / rho};
+        const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
         const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
+        const double h = (e + p) * inv_rho;
         // flux tensor F[c][x]
         double f[5][3];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
           f[0][x] = m[x];
-          f[1][x] = m[x] * (e + p) / rho;
+          f[1][x] = m[x] * h;
 #pragma unroll
-          for (int y = 0; y < 3; ++y) f[2 + y][x] = m[y] * m[x] / rho + (x == y ? p : 0.0);
+          for (int y = 0; y < 3; ++y) f[2 + y][x] = m[y] * u[x] + (x == y ? p : 0.0);
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c)
