@@ -112,6 +112,20 @@ __host__ __device__ constexpr int flux_items_of(int order) {
          flux_chunk_of(order) * 64;
 }
 
+// Whether the volume phase keeps a copy of the tile's state in LDS (as it
+// loads the B operands) so that the flux producer reads its own traces from
+// there instead of issuing NF more global loads per flux item.  It costs
+// NF * 16 * Np doubles of LDS per wave.  Measured on MI355X, it pays where the
+// blocks per CU stay what the registers allow (wave P1 -16%, wave P2 -9%,
+// Euler P3 -7%, Maxwell P1 -18%, Maxwell P3 -11%, Maxwell P2 even) and not
+// where the LDS halves them (wave P3 +8%, Euler P2 +18%); at P4 it does not fit.
+template <int EQ, int ORDER>
+constexpr bool stage_own_traces() {
+  if constexpr (EQ == DG_EQ_WAVE) return ORDER <= 2;
+  if constexpr (EQ == DG_EQ_EULER) return ORDER == 3;
+  return ORDER <= 3;
+}
+
 constexpr int kFluxRow = 65;                 // padded flux row (spreads LDS banks)
 
 // Per-wave LDS buffer (doubles): one flux chunk, or one field of the output tile.
@@ -158,6 +172,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   // per wave: scaled flux of C LIFT k-steps in MFMA B order [kk][field][FROW],
   // then one field of the output tile
   __shared__ double s_buf[WAVES][WB];
+  // per wave, if staged: the tile's state [field][n * NP + j]
+  constexpr bool ST = stage_own_traces<EQ, ORDER>();
+  __shared__ double s_stage[WAVES][ST ? NF * 16 * NP : 1];
+  double* const stg = s_stage[threadIdx.x >> 6];
 
   const int tid = threadIdx.x;
   for (int k = tid; k < S::OP_TILES * 64; k += kBlock) s_op[k] = a.op_tiles[k];
@@ -222,6 +240,12 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           vx = a.q.p[1][idx];
           vy = a.q.p[2][idx];
           vz = a.q.p[3][idx];
+          if constexpr (ST) {
+            stg[0 * 16 * NP + n * NP + j] = u;
+            stg[1 * 16 * NP + n * NP + j] = vx;
+            stg[2 * 16 * NP + n * NP + j] = vy;
+            stg[3 * 16 * NP + n * NP + j] = vz;
+          }
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -255,6 +279,12 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           const int64_t idx = el * NP + j;
           const double rho = a.q.p[0][idx], e = a.q.p[1][idx];
           const double m[3] = {a.q.p[2][idx], a.q.p[3][idx], a.q.p[4][idx]};
+          if constexpr (ST) {
+            stg[0 * 16 * NP + n * NP + j] = rho;
+            stg[1 * 16 * NP + n * NP + j] = e;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) stg[(2 + x) * 16 * NP + n * NP + j] = m[x];
+          }
           const double inv_rho = 1.0 / rho;
           const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
           const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
@@ -298,6 +328,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
         for (int ks = 0; ks < KS; ++ks) {
           const int j = g * KS + ks;
           const double u = j < NP ? a.q.p[c][el * NP + j] : 0.0;
+          if constexpr (ST)
+            if (j < NP) stg[c * 16 * NP + n * NP + j] = u;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -330,6 +362,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
 
     // {{{ surface terms, C LIFT k-steps per chunk
 
+    if constexpr (ST) wave_lds_fence();   // the staged state is complete
+
 #pragma unroll 1
     for (int ch = 0; ch < NCHUNK; ++ch) {
       // producer: items (face pair p, node kk) with kk fastest
@@ -350,8 +384,13 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           const int fn = fg * NFP + ks;
           const int64_t vn = pe * NP + s_fmask[fn];
           double qm[NF], qp[NF], out[NF];
+          if constexpr (ST) {
 #pragma unroll
-          for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
+            for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * NP + fe * NP + s_fmask[fn]];
+          } else {
+#pragma unroll
+            for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
+          }
           const int32_t ext =
               PF ? ext_pf[it] : a.ext_t[tile * flux_items_of(ORDER) + (ch * C + it) * 64 + lane];
           if (ext >= 0) {
