@@ -113,6 +113,20 @@ struct MixedShape {
 
 constexpr int kMixedFluxRow = 65;
 
+// Whether the volume phase keeps the tile's FP32 state in LDS (NF * 16 * Np
+// floats per wave) for the flux producer's own traces, as the FP64 MFMA
+// kernel does.  On where it fits without lowering the blocks per CU that the
+// registers allow and measured faster on MI355X: FP32 wave P1-P4 -4% to -8%,
+// Euler P1 -7%, P2 -16%, P4 -3%, Maxwell P1 -13%, P2 -4%, P3 -16%; BF16x3
+// wave P1 -3%, Euler P1-P3 -2% to -5% (BF16x3 Maxwell P1 and P3 were 13% and
+// 5% slower with it).
+template <int EQ, int PREC, int ORDER>
+constexpr bool mixed_stage_own_traces() {
+  if constexpr (PREC == DG_PREC_FP32)
+    return !((EQ == DG_EQ_EULER && ORDER == 3) || (EQ == DG_EQ_MAXWELL && ORDER == 4));
+  return (EQ == DG_EQ_WAVE && ORDER == 1) || (EQ == DG_EQ_EULER && ORDER <= 3);
+}
+
 template <int EQ, int PREC, int ORDER>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   using S = MixedShape<PREC, ORDER>;
@@ -130,6 +144,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   __shared__ uint32_t s_lift[S::LIFT_WORDS];
   __shared__ int s_fmask[NFN];
   __shared__ float s_buf[WAVES][WB];
+  constexpr bool ST = mixed_stage_own_traces<EQ, PREC, ORDER>();
+  __shared__ float s_stage[WAVES][ST ? NF * 16 * NP : 1];
+  float* const stg = s_stage[threadIdx.x >> 6];
 
   const int tid = threadIdx.x;
   {
@@ -189,6 +206,12 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
             vx = a.q[1][idx];
             vy = a.q[2][idx];
             vz = a.q[3][idx];
+            if constexpr (ST) {
+              stg[0 * 16 * NP + n * NP + j] = (float)u;
+              stg[1 * 16 * NP + n * NP + j] = (float)vx;
+              stg[2 * 16 * NP + n * NP + j] = (float)vy;
+              stg[3 * 16 * NP + n * NP + j] = (float)vz;
+            }
           }
           ub[i] = (float)u;
 #pragma unroll
@@ -217,6 +240,12 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
             const int64_t idx = el * NP + j;
             const double rho = a.q[0][idx], e = a.q[1][idx];
             const double m[3] = {a.q[2][idx], a.q[3][idx], a.q[4][idx]};
+            if constexpr (ST) {
+              stg[0 * 16 * NP + n * NP + j] = (float)rho;
+              stg[1 * 16 * NP + n * NP + j] = (float)e;
+#pragma unroll
+              for (int x = 0; x < 3; ++x) stg[(2 + x) * 16 * NP + n * NP + j] = (float)m[x];
+            }
             const double inv_rho = 1.0 / rho;
             const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
             const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
@@ -280,6 +309,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
           for (int i = 0; i < KG; ++i) {
             const int j = g * KS + kq * KG + i;
             ub[i] = j < NP ? a.q[c][el * NP + j] : 0.f;
+            if constexpr (ST)
+              if (j < NP) stg[c * 16 * NP + n * NP + j] = ub[i];
           }
 #pragma unroll
           for (int r = 0; r < 3; ++r)
@@ -319,6 +350,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 
     // {{{ surface terms: flux in FP64, staged as FP32 B operands
 
+    if constexpr (ST) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // staged state complete
+
 #pragma unroll
     for (int it = 0; it < KSL; ++it) {
       const int item = it * 64 + lane;
@@ -336,7 +369,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         const int64_t vn = pe * NP + s_fmask[fn];
         double qm[NF], qp[NF], out[NF];
 #pragma unroll
-        for (int c = 0; c < NF; ++c) qm[c] = a.q[c][vn];
+        for (int c = 0; c < NF; ++c)
+          qm[c] = ST ? (double)stg[c * 16 * NP + fe * NP + s_fmask[fn]] : (double)a.q[c][vn];
         const int32_t ext = a.emap[pe * NFN + fn];
         if (ext >= 0) {
 #pragma unroll
