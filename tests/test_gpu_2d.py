@@ -68,6 +68,17 @@ def test_rhs_2d_matches_oracle_many_batches(equation, order, nel, device):
     assert max(errs) < TOL_FP64, errs
 
 
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+@pytest.mark.parametrize("order", [1, 4])
+def test_rhs_2d_two_triangles(equation, order, device):
+    """One square cell: two triangles, every edge but the diagonal on the wall."""
+    d, od = discr_pair_2d(order, 1)
+    q = random_state(equation, od.x, seed=3)
+    plan = DGPlan(d, equation, PARAM[equation])
+    got = _run(plan, q, device)
+    assert max(field_errors(got, oracle_rhs(equation, od, q, PARAM[equation]))) < TOL_FP64
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
 def test_rhs_2d_fp32_storage(equation, precision, device):

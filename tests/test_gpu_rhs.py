@@ -200,3 +200,27 @@ def test_plan_rejects_bad_fields(device):
         plan.rhs(good, precision="fp16")
 
 # }}}
+
+
+@pytest.mark.parametrize("variant", ["mfma", "lds"])
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_rhs_single_partial_tile(equation, order, variant, device):
+    """A mesh smaller than one 16-element MFMA tile (one hexahedron, 6 tets):
+    the padded tile, the tile-ordered tables and the staged state all see
+    fewer elements than their slots."""
+    print(f"[case] rhs {variant} {equation} P{order} n=1", flush=True)
+    d, od = discr_pair(order, 1)
+    q = random_state(equation, od.x, seed=13)
+    plan = DGPlan(d, equation, PARAM[equation], variant=variant)
+    got = _run(plan, q, device)
+    ref = oracle_rhs(equation, od, q, PARAM[equation])
+    assert max(field_errors(got, ref)) < TOL_FP64
+    if variant == "mfma":
+        # the mixed form on the same partial tile
+        qt = [torch.from_numpy(np.ascontiguousarray(f)).to(device).float() for f in q]
+        for prec in ("fp32", "bf16x3"):
+            mixed = plan.rhs(qt, precision=prec)
+            torch.cuda.synchronize()
+            m = np.stack([t.cpu().numpy().astype(np.float64) for t in mixed])
+            assert max(field_errors(m, ref)) < 1e-3, prec
