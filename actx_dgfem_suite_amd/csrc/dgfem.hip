@@ -310,34 +310,44 @@ using VjpKernel = void (*)(VjpArgs<T>);
 template <typename T>
 using MassInvKernel = void (*)(MassInvArgs<T>);
 
-template <int EQ, typename T>
+template <int EQ, typename T, int DIM>
 VjpKernel<T> pick_vjp_order(int order) {
   switch (order) {
-    case 1: return rhs_vjp_kernel<EQ, 1, T>;
-    case 2: return rhs_vjp_kernel<EQ, 2, T>;
-    case 3: return rhs_vjp_kernel<EQ, 3, T>;
-    case 4: return rhs_vjp_kernel<EQ, 4, T>;
+    case 1: return rhs_vjp_kernel<EQ, 1, T, DIM>;
+    case 2: return rhs_vjp_kernel<EQ, 2, T, DIM>;
+    case 3: return rhs_vjp_kernel<EQ, 3, T, DIM>;
+    case 4: return rhs_vjp_kernel<EQ, 4, T, DIM>;
     default: return nullptr;
   }
 }
 
-template <typename T>
-VjpKernel<T> pick_vjp(int eq, int order) {
+template <typename T, int DIM>
+VjpKernel<T> pick_vjp_dim(int eq, int order) {
   switch (eq) {
-    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE, T>(order);
-    case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER, T>(order);
-    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL, T>(order);
+    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE, T, DIM>(order);
+    case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER, T, DIM>(order);
+    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL, T, DIM>(order);
     default: return nullptr;
   }
 }
 
 template <typename T>
-MassInvKernel<T> pick_mass_inv(int order) {
-  switch (order) {
-    case 1: return mass_inv_kernel<1, T>;
-    case 2: return mass_inv_kernel<2, T>;
+VjpKernel<T> pick_vjp(int eq, int order, int dim) {
+  return dim == 2 ? pick_vjp_dim<T, 2>(eq, order) : pick_vjp_dim<T, 3>(eq, order);
+}
+
+// M^{-1} kernels by node count (tetrahedra: 4, 10, 20, 35; triangles: 3, 6,
+// 10, 15)
+template <typename T>
+MassInvKernel<T> pick_mass_inv(int np) {
+  switch (np) {
     case 3: return mass_inv_kernel<3, T>;
     case 4: return mass_inv_kernel<4, T>;
+    case 6: return mass_inv_kernel<6, T>;
+    case 10: return mass_inv_kernel<10, T>;
+    case 15: return mass_inv_kernel<15, T>;
+    case 20: return mass_inv_kernel<20, T>;
+    case 35: return mass_inv_kernel<35, T>;
     default: return nullptr;
   }
 }
@@ -379,9 +389,12 @@ bool invert(int n, const double* a, std::vector<double>& inv) {
 // of every volume node, and the neighbour's Fscale across every face.  Fails
 // (returns a message) when E couples nodes off its face or the exterior map
 // does not pair whole faces.
-std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp,
-                               std::vector<double>& face_mass, std::vector<int32_t>& node_faces,
-                               std::vector<double>& ext_fscale) {
+// nfaces faces per element; geometry rows of geo_stride doubles with each
+// face's Fscale at fs_offset + fs_stride * f.
+std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp, int nfaces, int geo_stride,
+                               int fs_offset, int fs_stride, std::vector<double>& face_mass,
+                               std::vector<int32_t>& node_faces, std::vector<double>& ext_fscale) {
+  const int kFaces = nfaces;
   const int nfn = kFaces * nfp;
   const int64_t k = d->nelements;
   const int* fmask = d->face_nodes;
@@ -423,7 +436,7 @@ std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp,
         continue;
       }
       const int64_t e2 = em[0] / np;
-      int mask = 0xF;
+      int mask = (1 << kFaces) - 1;
       for (int a = 0; a < nfp; ++a) {
         if (em[a] < 0 || em[a] / np != e2) return "ext_map does not pair whole faces";
         mask &= bits[em[a] % np];
@@ -432,7 +445,7 @@ std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp,
         return "ext_map: exterior nodes of a face do not lie on one face";
       int f2 = 0;
       while (!(mask & (1 << f2))) ++f2;
-      ext_fscale[e * kFaces + f] = d->geometry[e2 * kGeo + 9 + 4 * f2 + 3];
+      ext_fscale[e * kFaces + f] = d->geometry[e2 * geo_stride + fs_offset + fs_stride * f2];
     }
   return "";
 }
@@ -641,10 +654,10 @@ struct dg_plan {
   int vjp_grid = 0;
   int vjp_grid32 = 0;
   int massinv_grid = 0;
-  double* amat_t = nullptr;      // (3, Np, Np): A_r = D_r^T M at [r][j][i]
-  double* face_mass = nullptr;   // (4, Nfp, Nfp)
+  double* amat_t = nullptr;      // (d, Np, Np): A_r = D_r^T M at [r][j][i]
+  double* face_mass = nullptr;   // (F, Nfp, Nfp)
   double* mass_inv = nullptr;    // (Np, Np)
-  double* ext_fscale = nullptr;  // (K, 4)
+  double* ext_fscale = nullptr;  // (K, F)
   int32_t* node_faces = nullptr; // (Np, 3)
 };
 
@@ -812,8 +825,6 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   if (tri && d->variant == DG_KERNEL_MFMA)
     return fail(DG_ERR_UNSUPPORTED, "MFMA kernel: tetrahedral plans only (dim 2 runs the "
                                     "LDS-staged kernel)");
-  if (tri && (d->mass_inv || d->face_mass))
-    return fail(DG_ERR_UNSUPPORTED, "adjoint (mass_inv / face_mass): tetrahedral plans only");
 
   const int64_t k = d->nelements;
   const int nfn = nfaces * nfp;
@@ -834,15 +845,17 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   if (want_vjp) {
     if (!d->mass_inv || !d->face_mass)
       return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv and face_mass must be given together");
-    const std::string msg = build_adjoint_data(d, np, nfp, fm, nf, xfs);
+    const std::string msg =
+        tri ? build_adjoint_data(d, np, nfp, kFaces2, kGeo2, 4 + 2, 3, fm, nf, xfs)
+            : build_adjoint_data(d, np, nfp, kFaces, kGeo, 9 + 3, 4, fm, nf, xfs);
     if (!msg.empty()) return fail(DG_ERR_INVALID_ARGUMENT, msg);
     // A_r = D_r^T M, stored [r][j][i] = A_r[i][j]: the adjoint's volume
     // operator acting on mu = M^{-1} lam
     std::vector<double> mass;
     if (!invert(np, d->mass_inv, mass)) return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv is singular");
     const double* dop = d->equation == DG_EQ_MAXWELL ? d->d_strong : d->d_weak;
-    amat_t.assign((size_t)3 * np * np, 0.0);
-    for (int r = 0; r < 3; ++r)
+    amat_t.assign((size_t)ndir * np * np, 0.0);
+    for (int r = 0; r < ndir; ++r)
       for (int i = 0; i < np; ++i)
         for (int j = 0; j < np; ++j) {
           double acc = 0.0;
@@ -983,10 +996,10 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       free_plan(p);
       return rc;
     }
-    p->vjp_kernel = pick_vjp<double>(d->equation, d->order);
-    p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order);
-    p->massinv_kernel = pick_mass_inv<double>(d->order);
-    p->massinv_kernel32 = pick_mass_inv<float>(d->order);
+    p->vjp_kernel = pick_vjp<double>(d->equation, d->order, p->dim);
+    p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order, p->dim);
+    p->massinv_kernel = pick_mass_inv<double>(np);
+    p->massinv_kernel32 = pick_mass_inv<float>(np);
     const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
     auto grid_of = [&](const void* fn) {
       int occ = 0;
@@ -1112,8 +1125,6 @@ template <typename T>
 int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T* const* q_bar,
                  T* const* workspace, void* stream, const char* name) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
-  if (p->dim == 2)
-    return fail(DG_ERR_UNSUPPORTED, std::string(name) + ": adjoint: tetrahedral plans only");
   if (!p->has_vjp)
     return fail(DG_ERR_UNSUPPORTED, std::string(name) +
                                         ": plan created without mass_inv / face_mass: no adjoint");

@@ -33,6 +33,7 @@
 #pragma once
 
 #include "dg_common.h"
+#include "rhs_2d.h"
 
 namespace dg {
 
@@ -208,6 +209,141 @@ template <> struct AdjOf<DG_EQ_WAVE> { using type = WaveAdj; };
 template <> struct AdjOf<DG_EQ_EULER> { using type = EulerAdj; };
 template <> struct AdjOf<DG_EQ_MAXWELL> { using type = MaxwellAdj; };
 
+// {{{ the same for the triangle fluxes of rhs_2d.h
+
+// Wave2DFlux: out = c({v}.n + [u]/2, ({u} + [v].n/2) n) with two components.
+struct Wave2DAdj {
+  static constexpr int NF = 3;
+  static constexpr bool CONSERVATIVE = true;
+  __device__ static void vjp_m(const double*, const double*, const double* n, double c,
+                               const double* psi, double* gM) {
+    const double pvn = psi[1] * n[0] + psi[2] * n[1];
+    gM[0] = 0.5 * c * (-psi[0] + pvn);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) gM[1 + k] = 0.5 * c * (psi[0] - pvn) * n[k];
+  }
+  __device__ static void vjp(const double*, const double*, const double* n, double c,
+                             const double* psi, double* gM, double* gP) {
+    const double pvn = psi[1] * n[0] + psi[2] * n[1];
+    gM[0] = 0.5 * c * (-psi[0] + pvn);
+    gP[0] = 0.5 * c * (psi[0] + pvn);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      gM[1 + k] = 0.5 * c * (psi[0] - pvn) * n[k];
+      gP[1 + k] = 0.5 * c * (psi[0] + pvn) * n[k];
+    }
+  }
+  __device__ static void bcT(const double*, double* g) { g[0] = -g[0]; }
+};
+
+// Maxwell2DFlux (TM): out depends on the jump qP - qM only, through
+// d = [E_z] and w = n_x [H_y] - n_y [H_x]:
+//   out = (w/2 + alpha d/2, -n_y (d + alpha w)/2, n_x (d + alpha w)/2).
+// With a = n_x psi_Hy - n_y psi_Hx:
+//   gP = (alpha psi_Ez/2 + a/2, -n_y (psi_Ez + alpha a)/2, n_x (psi_Ez + alpha a)/2), gM = -gP.
+struct Maxwell2DAdj {
+  static constexpr int NF = 3;
+  static constexpr bool CONSERVATIVE = false;
+  __device__ static void vjp_m(const double*, const double*, const double*, double,
+                               const double*, double*) {}
+  __device__ static void vjp(const double*, const double*, const double* n, double alpha,
+                             const double* psi, double* gM, double* gP) {
+    const double av = n[0] * psi[2] - n[1] * psi[1];
+    const double t = psi[0] + alpha * av;
+    gP[0] = 0.5 * alpha * psi[0] + 0.5 * av;
+    gP[1] = -0.5 * n[1] * t;
+    gP[2] = 0.5 * n[0] * t;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gM[c] = -gP[c];
+  }
+  __device__ static void bcT(const double*, double* g) { g[0] = -g[0]; }
+};
+
+// Euler2DFlux: LF flux with fields (mass, energy, momentum[2]).
+struct Euler2DAdj {
+  static constexpr int NF = 4;
+  static constexpr bool CONSERVATIVE = true;
+  // g += (d (F(q).n) / d q)^T psi, n need not be a unit vector
+  __device__ static void nflux_vjp(const double* q, const double* n, double gamma,
+                                   const double* psi, double* g) {
+    const double rho = q[0], e = q[1];
+    const double m[2] = {q[2], q[3]};
+    const double inv = 1.0 / rho;
+    const double m2 = m[0] * m[0] + m[1] * m[1];
+    const double p = (gamma - 1.0) * (e - 0.5 * m2 * inv);
+    const double un = (m[0] * n[0] + m[1] * n[1]) * inv;
+    const double pmn = psi[2] * n[0] + psi[3] * n[1];
+    const double pmm = psi[2] * m[0] + psi[3] * m[1];
+    const double Q = psi[1] * (e + p) + pmm;
+    const double R = psi[1] * un + pmn;
+    g[0] += R * (gamma - 1.0) * 0.5 * m2 * inv * inv - Q * un * inv;
+    g[1] += psi[1] * un + (gamma - 1.0) * R;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      g[2 + k] += psi[0] * n[k] + Q * inv * n[k] + un * psi[2 + k] - (gamma - 1.0) * R * m[k] * inv;
+  }
+  // wavespeed |u| + c and its gradient
+  __device__ static double speed(const double* q, double gamma, double* ds) {
+    const double rho = q[0], e = q[1];
+    const double inv = 1.0 / rho;
+    const double u[2] = {q[2] * inv, q[3] * inv};
+    const double uu = u[0] * u[0] + u[1] * u[1];
+    const double p = (gamma - 1.0) * (e - 0.5 * rho * uu);
+    const double c = sqrt(gamma * p * inv);
+    const double au = sqrt(uu);
+    const double dp_drho = (gamma - 1.0) * 0.5 * uu;
+    const double k = gamma / (2.0 * c * rho);
+    ds[0] = (au > 0.0 ? -au * inv : 0.0) + k * (dp_drho - p * inv);
+    ds[1] = k * (gamma - 1.0);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      ds[2 + j] = (au > 0.0 ? u[j] / (au * rho) : 0.0) - k * (gamma - 1.0) * u[j];
+    return au + c;
+  }
+  template <bool NEED_P>
+  __device__ static void vjp_impl(const double* qM, const double* qP, const double* n,
+                                  double gamma, const double* psi, double* gM, double* gP) {
+    double dsM[4], dsP[4];
+    const double lM = speed(qM, gamma, dsM);
+    const double lP = speed(qP, gamma, dsP);
+    const double lam = fmax(lM, lP);
+    const double wM = lM > lP ? 1.0 : (lM < lP ? 0.0 : 0.5);
+    const double wP = 1.0 - wM;
+    double jump = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) jump += psi[c] * (qP[c] - qM[c]);
+    double hpsi[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      hpsi[c] = -0.5 * psi[c];
+      gM[c] = -0.5 * lam * psi[c] + 0.5 * jump * wM * dsM[c];
+      if constexpr (NEED_P) gP[c] = 0.5 * lam * psi[c] + 0.5 * jump * wP * dsP[c];
+    }
+    nflux_vjp(qM, n, gamma, hpsi, gM);
+    if constexpr (NEED_P) nflux_vjp(qP, n, gamma, hpsi, gP);
+  }
+  __device__ static void vjp(const double* qM, const double* qP, const double* n, double gamma,
+                             const double* psi, double* gM, double* gP) {
+    vjp_impl<true>(qM, qP, n, gamma, psi, gM, gP);
+  }
+  __device__ static void vjp_m(const double* qM, const double* qP, const double* n, double gamma,
+                               const double* psi, double* gM) {
+    vjp_impl<false>(qM, qP, n, gamma, psi, gM, nullptr);
+  }
+  __device__ static void bcT(const double* n, double* g) {
+    const double gn = g[2] * n[0] + g[3] * n[1];
+    g[2] -= 2.0 * gn * n[0];
+    g[3] -= 2.0 * gn * n[1];
+  }
+};
+
+template <int EQ> struct Adj2DOf;
+template <> struct Adj2DOf<DG_EQ_WAVE> { using type = Wave2DAdj; };
+template <> struct Adj2DOf<DG_EQ_EULER> { using type = Euler2DAdj; };
+template <> struct Adj2DOf<DG_EQ_MAXWELL> { using type = Maxwell2DAdj; };
+
+// }}}
+
 // }}}
 
 // {{{ mu = M^{-1} lam per element: batches of kBlock / Np elements; every
@@ -216,9 +352,8 @@ template <> struct AdjOf<DG_EQ_MAXWELL> { using type = MaxwellAdj; };
 // persistent grid of several blocks per CU keeps loads of many batches in
 // flight.
 
-template <int ORDER, typename T>
+template <int NP, typename T>
 __global__ __launch_bounds__(kBlock) void mass_inv_kernel(MassInvArgs<T> a) {
-  constexpr int NP = np_of(ORDER);
   constexpr int EB = kBlock / NP;
   __shared__ double s_m[NP * NP];     // s_m[j * NP + i] = Minv[i][j] (= Minv[j][i])
   __shared__ double s_l[kMaxFields][EB * NP];
@@ -248,21 +383,48 @@ __global__ __launch_bounds__(kBlock) void mass_inv_kernel(MassInvArgs<T> a) {
 
 // {{{ adjoint kernel: one thread per (element, node), element batches in LDS
 
-template <int EQ, int ORDER, typename T>
+// Shapes of the adjoint kernel on tetrahedra (DIM 3) and triangles (DIM 2):
+// geometry rows hold rx (DIM * DIM), then per face DIM normal components and
+// Fscale.
+template <int DIM>
+struct AdjShape {
+  static constexpr int FACES = DIM + 1;
+  static constexpr int GEO = DIM == 3 ? kGeo : kGeo2;
+  static constexpr int RX = DIM * DIM;
+  static constexpr int FACE_STRIDE = DIM + 1;
+  static constexpr int np(int order) { return DIM == 3 ? np_of(order) : np2_of(order); }
+  static constexpr int nfp(int order) { return DIM == 3 ? nfp_of(order) : nfp2_of(order); }
+  static constexpr int nfields(int eq) { return DIM == 3 ? nfields_of(eq) : nfields2_of(eq); }
+};
+
+template <int EQ, int DIM> struct AdjFluxOf;
+template <int EQ> struct AdjFluxOf<EQ, 3> {
+  using adj = typename AdjOf<EQ>::type;
+  using flux = typename FluxOf<EQ>::type;
+};
+template <int EQ> struct AdjFluxOf<EQ, 2> {
+  using adj = typename Adj2DOf<EQ>::type;
+  using flux = typename Flux2DOf<EQ>::type;
+};
+
+template <int EQ, int ORDER, typename T, int DIM = 3>
 __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
-  using Adj = typename AdjOf<EQ>::type;
-  using Flux = typename FluxOf<EQ>::type;
-  constexpr int NP = np_of(ORDER);
-  constexpr int NFP = nfp_of(ORDER);
-  constexpr int NF = nfields_of(EQ);
-  constexpr int NFN = kFaces * NFP;
+  using Sh = AdjShape<DIM>;
+  using Adj = typename AdjFluxOf<EQ, DIM>::adj;
+  using Flux = typename AdjFluxOf<EQ, DIM>::flux;
+  constexpr int NP = Sh::np(ORDER);
+  constexpr int NFP = Sh::nfp(ORDER);
+  constexpr int NF = Sh::nfields(EQ);
+  constexpr int NFACES = Sh::FACES;
+  constexpr int GEO = Sh::GEO;
+  constexpr int NFN = NFACES * NFP;
   constexpr int EB = kBlock / NP;
   constexpr bool NEEDS_Q = EQ == DG_EQ_EULER;
   constexpr int NQ = NEEDS_Q ? NF : 1;
-  constexpr int NW = EQ == DG_EQ_WAVE ? 3 : 1;
+  constexpr int NW = EQ == DG_EQ_WAVE ? DIM : 1;
 
-  __shared__ double s_a[3 * NP * NP];        // A_r[i][j] at [r][j][i]
-  __shared__ double s_mf[kFaces * NFP * NFP];
+  __shared__ double s_a[DIM * NP * NP];      // A_r[i][j] at [r][j][i]
+  __shared__ double s_mf[NFACES * NFP * NFP];
   __shared__ int s_fmask[NFN];
   __shared__ int s_nf[NP * 3];
   __shared__ double s_mu[NF][EB * NP];
@@ -272,12 +434,12 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
   // q at exterior nodes (Euler), overwritten in place (same thread, same
   // slot) by the face-node contributions to q_bar
   __shared__ double s_x[NF][EB * NFN];
-  __shared__ double s_geo[EB * kGeo];
-  __shared__ double s_xfs[EB * kFaces];
+  __shared__ double s_geo[EB * GEO];
+  __shared__ double s_xfs[EB * NFACES];
 
   const int tid = threadIdx.x;
-  for (int k = tid; k < 3 * NP * NP; k += kBlock) s_a[k] = a.amat_t[k];
-  for (int k = tid; k < kFaces * NFP * NFP; k += kBlock) s_mf[k] = a.face_mass[k];
+  for (int k = tid; k < DIM * NP * NP; k += kBlock) s_a[k] = a.amat_t[k];
+  for (int k = tid; k < NFACES * NFP * NFP; k += kBlock) s_mf[k] = a.face_mass[k];
   for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
   for (int k = tid; k < NP * 3; k += kBlock) s_nf[k] = a.node_faces[k];
 
@@ -346,19 +508,24 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
         s_mu[c][k] = (double)a.mu[c][e0 * NP + k];
         if constexpr (NEEDS_Q) s_q[c][k] = (double)a.q[c][e0 * NP + k];
       }
-    for (int k = tid; k < nel * kGeo; k += kBlock) s_geo[k] = a.geo[e0 * kGeo + k];
-    for (int k = tid; k < nel * kFaces; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * kFaces + k];
+    for (int k = tid; k < nel * GEO; k += kBlock) s_geo[k] = a.geo[e0 * GEO + k];
+    for (int k = tid; k < nel * NFACES; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * NFACES + k];
     if (has_next) issue_gathers();
     __syncthreads();
 
     const bool active = el < nel;
     if constexpr (EQ == DG_EQ_WAVE) {
       if (active) {
-        const double* g = &s_geo[el * kGeo];
+        const double* g = &s_geo[el * GEO];
         const int node = el * NP + i;
+        if constexpr (DIM == 3) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-          s_w[r][node] = g[r] * s_mu[1][node] + g[3 + r] * s_mu[2][node] + g[6 + r] * s_mu[3][node];
+          for (int r = 0; r < 3; ++r)
+            s_w[r][node] = g[r] * s_mu[1][node] + g[3 + r] * s_mu[2][node] + g[6 + r] * s_mu[3][node];
+        } else {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) s_w[r][node] = g[r] * s_mu[1][node] + g[2 + r] * s_mu[2][node];
+        }
       }
     }
 
@@ -368,10 +535,12 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
       const int fn = k - fe * NFN;
       const int f = fn / NFP;
       const int aa = fn - f * NFP;
-      const double* geo = &s_geo[fe * kGeo + 9 + 4 * f];
-      const double n[3] = {geo[0], geo[1], geo[2]};
-      const double fs = geo[3];
-      const double xfs = s_xfs[fe * kFaces + f];
+      const double* geo = &s_geo[fe * GEO + Sh::RX + Sh::FACE_STRIDE * f];
+      double n[DIM];
+#pragma unroll
+      for (int x = 0; x < DIM; ++x) n[x] = geo[x];
+      const double fs = geo[DIM];
+      const double xfs = s_xfs[fe * NFACES + f];
       double psi[NF], psin[NF];
 #pragma unroll
       for (int c = 0; c < NF; ++c) {
@@ -403,7 +572,9 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
         for (int c = 0; c < NF; ++c) pd[c] = psi[c] - psin[c];
         Adj::vjp_m(qm, qp, n, c0, pd, gM);
       } else {
-        const double nm[3] = {-n[0], -n[1], -n[2]};
+        double nm[DIM];
+#pragma unroll
+        for (int x = 0; x < DIM; ++x) nm[x] = -n[x];
         Adj::vjp(qm, qp, n, c0, psi, gM, gP);
         double hM[NF], hP[NF];
         // the neighbour's flux: its interior is qp, its exterior is our qm
@@ -417,10 +588,68 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
     __syncthreads();
 
     if (active) {
-      const double* g = &s_geo[el * kGeo];
+      const double* g = &s_geo[el * GEO];
       const int base = el * NP;
       double acc[NF];
-      if constexpr (EQ == DG_EQ_WAVE) {
+      if constexpr (DIM == 2) {
+        // t_{c,r} = A_r mu_c (wave: the weak divergence's transpose on w)
+        if constexpr (EQ == DG_EQ_WAVE) {
+          double au = 0.0, t0 = 0.0, t1 = 0.0;
+          for (int j = 0; j < NP; ++j) {
+            const double d0 = s_a[j * NP + i];
+            const double d1 = s_a[NP * NP + j * NP + i];
+            const double mu = s_mu[0][base + j];
+            au += d0 * s_w[0][base + j] + d1 * s_w[1][base + j];
+            t0 += d0 * mu;
+            t1 += d1 * mu;
+          }
+          acc[0] = -c0 * au;
+#pragma unroll
+          for (int x = 0; x < 2; ++x) acc[1 + x] = -c0 * (g[2 * x] * t0 + g[2 * x + 1] * t1);
+        } else {
+          double t[NF][2];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) t[c][0] = t[c][1] = 0.0;
+          for (int j = 0; j < NP; ++j) {
+            const double d0 = s_a[j * NP + i];
+            const double d1 = s_a[NP * NP + j * NP + i];
+#pragma unroll
+            for (int c = 0; c < NF; ++c) {
+              const double l = s_mu[c][base + j];
+              t[c][0] += d0 * l;
+              t[c][1] += d1 * l;
+            }
+          }
+          // d[c][x] = sum_r rx[x][r] t[c][r]
+          double d[NF][2];
+#pragma unroll
+          for (int c = 0; c < NF; ++c)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) d[c][x] = g[2 * x] * t[c][0] + g[2 * x + 1] * t[c][1];
+          if constexpr (EQ == DG_EQ_EULER) {
+            double q[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              q[c] = s_q[c][base + i];
+              acc[c] = 0.0;
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+              double fb[4];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) fb[c] = d[c][x];
+              const double ex[2] = {x == 0 ? 1.0 : 0.0, x == 1 ? 1.0 : 0.0};
+              Euler2DAdj::nflux_vjp(q, ex, c0, fb, acc);
+            }
+          } else {
+            // TM transpose: E_z_bar = dx^T lam_Hy - dy^T lam_Hx,
+            // H_x_bar = -dy^T lam_Ez, H_y_bar = dx^T lam_Ez
+            acc[0] = d[2][0] - d[1][1];
+            acc[1] = -d[0][1];
+            acc[2] = d[0][0];
+          }
+        }
+      } else if constexpr (EQ == DG_EQ_WAVE) {
         // u_bar = -c sum_r A_r (sum_x rx[x][r] mu_vx); v_x_bar = -c sum_r rx[x][r] A_r mu_u
         double au = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
         for (int j = 0; j < NP; ++j) {

@@ -34,7 +34,7 @@ class DGPlan:
     equations/maxwell.py:252-254.  ``variant`` picks the FP64 kernel:
     ``"auto"``/``"mfma"`` (matrix cores) or ``"lds"`` (the baseline).
     Triangle discretizations (``discr.dim == 2``) run the LDS-staged 2D kernel
-    in FP64; their plans have no mixed-precision or adjoint form.
+    (FP64 arithmetic; the mixed precisions store FP32 fields) and its adjoint.
     """
 
     def __init__(self, discr: Discretization, equation: str, param: float,
@@ -83,10 +83,8 @@ class DGPlan:
         desc.face_nodes = _dptr(h["face_nodes"], ctypes.c_int32)
         desc.geometry = _dptr(h["geometry"], ctypes.c_double)
         desc.ext_map = _dptr(h["ext_map"], ctypes.c_int32)
-        if self.dim == 3:
-            # adjoint data (tetrahedra only)
-            desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
-            desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
+        desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
+        desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
         desc.dim = self.dim
         desc.centroids = _dptr(h["centroids"], ctypes.c_double)
         desc.params[0] = self.param

@@ -89,10 +89,10 @@ typedef struct dg_plan_desc {
   double params[4];
   int32_t variant;           /* dg_kernel_variant (triangles: AUTO or LDS) */
   int32_t dim;               /* 3 (or 0): tetrahedra; 2: triangles, which run
-                              * the LDS-staged kernel, FP64 only, no adjoint */
+                              * the LDS-staged kernel with FP64 arithmetic */
   /* Needed by dg_rhs_vjp only (both NULL for a forward-only plan):
-   * (Np, Np) reference inverse mass M^{-1}, and (Np, 4*Nfp) reference face
-   * mass E with LIFT = M^{-1} E (mass_inverse_fuser.py:21-38 is the fusion
+   * (Np, Np) reference inverse mass M^{-1}, and (Np, F*Nfp) reference face
+   * mass E with LIFT = M^{-1} E (F = 4 faces, 3 edges for triangles) (mass_inverse_fuser.py:21-38 is the fusion
    * the adjoint splits apart again). */
   const double* mass_inv;
   const double* face_mass;

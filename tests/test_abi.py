@@ -113,7 +113,7 @@ def test_plan_create_checks_adjoint_data_without_touching_the_gpu():
 
 
 def _small_desc_2d(keep):
-    """A valid descriptor of a 3^2 x 2 triangle P2 mesh (no adjoint data)."""
+    """A valid descriptor of a 3^2 x 2 triangle P2 mesh with the adjoint data."""
     import numpy as np
 
     from actx_dgfem_suite_amd.discretization.triangle import make_square_discretization
@@ -126,6 +126,8 @@ def _small_desc_2d(keep):
         "face_nodes": np.array(d.ref.face_nodes, dtype=np.int32, order="C"),
         "geometry": np.array(d.packed_geometry(), dtype=np.float64, order="C"),
         "ext_map": np.array(d.packed_ext_map(), dtype=np.int32, order="C"),
+        "mass_inv": np.array(d.ref.mass_inv, dtype=np.float64, order="C"),
+        "face_mass": np.array(d.ref.face_mass, dtype=np.float64, order="C"),
     }
     keep.update(arrs)
     desc = _lib.DgPlanDesc()
@@ -142,9 +144,8 @@ def _small_desc_2d(keep):
 
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libdgfem.so not built")
 def test_plan_create_checks_triangle_descriptors_without_touching_the_gpu():
-    """dim selects triangles; their plans reject the MFMA kernel and the
-    adjoint data, and their exterior map is range-checked with 3 faces."""
-    import numpy as np
+    """dim selects triangles; their plans reject the MFMA kernel, and their
+    exterior map and adjoint data are checked with 3 faces of 3 nodes."""
     lib = _lib.load_library()
     handle = ctypes.c_void_p()
     keep = {}
@@ -157,11 +158,17 @@ def test_plan_create_checks_triangle_descriptors_without_touching_the_gpu():
     assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 2
     assert b"MFMA" in lib.dg_last_error_string()
     desc = _small_desc_2d(keep)
-    mi = np.eye(6)
-    keep["mi"] = mi
-    desc.mass_inv = mi.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 2
-    assert b"adjoint" in lib.dg_last_error_string()
+    desc.face_mass = None
+    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 1
+    assert b"together" in lib.dg_last_error_string()
+    desc = _small_desc_2d(keep)
+    fm = keep["face_mass"]
+    assert fm.shape == (6, 9)      # (Np, 3 faces x Nfp)
+    fnodes = set(keep["face_nodes"][2].tolist())
+    off = next(i for i in range(6) if i not in fnodes)
+    fm[off, 2 * 3] = 0.5           # face 2 couples a node off that edge
+    assert lib.dg_plan_create(ctypes.byref(desc), ctypes.byref(handle)) == 1
+    assert b"off its face" in lib.dg_last_error_string()
     desc = _small_desc_2d(keep)
     em = keep["ext_map"]
     assert em.shape == (18, 9)
