@@ -127,6 +127,21 @@ constexpr bool mixed_stage_own_traces() {
   return (EQ == DG_EQ_WAVE && ORDER == 1) || (EQ == DG_EQ_EULER && ORDER <= 3);
 }
 
+// Whether the flux producer gathers without a branch (a boundary node loads
+// its own trace and takes the boundary state by selection), so that all items
+// are one basic block and their loads are in flight together, as in the FP64
+// MFMA kernel.  Measured on MI355X, bit-identical results: FP32 wave P3 -12%,
+// P4 -17%, Euler P3 -7%, P4 -3%, Maxwell P1-P4 -7% to -11%; BF16x3 wave P1
+// -7%, P4 -15%, Euler P1-P4 -2% to -10%, Maxwell P1 -6%, P2 -2%.  Off where
+// it was slower: FP32 wave P1 (+2%), P2 (+15%) and Euler P2 (+25%, where its
+// registers cost a wave per SIMD); BF16x3 Maxwell P3 (+25%) and P4 (+17%).
+template <int EQ, int PREC, int ORDER>
+constexpr bool mixed_branch_free_gathers() {
+  if constexpr (PREC == DG_PREC_FP32)
+    return !((EQ == DG_EQ_WAVE && ORDER <= 2) || (EQ == DG_EQ_EULER && ORDER == 2));
+  return !(EQ == DG_EQ_MAXWELL && ORDER >= 3);
+}
+
 template <int EQ, int PREC, int ORDER>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   using S = MixedShape<PREC, ORDER>;
@@ -372,7 +387,15 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         for (int c = 0; c < NF; ++c)
           qm[c] = ST ? (double)stg[c * 16 * NP + fe * NP + s_fmask[fn]] : (double)a.q[c][vn];
         const int32_t ext = a.emap[pe * NFN + fn];
-        if (ext >= 0) {
+        if constexpr (mixed_branch_free_gathers<EQ, PREC, ORDER>()) {
+          const int64_t src = ext >= 0 ? (int64_t)ext : vn;
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qp[c] = a.q[c][src];
+          double qb[NF];
+          Flux::boundary(qm, nrm, qb);
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+        } else if (ext >= 0) {
 #pragma unroll
           for (int c = 0; c < NF; ++c) qp[c] = a.q[c][ext];
         } else {
