@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "dg_lserk_stage_adjoint",
     "dg_rhs_vjp_fp32",
     "dg_lserk_stage_adjoint_fp32",
+    "dg_adjoint_space",
     "dg_last_error_string",
 )
 
@@ -103,6 +104,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                                 ctypes.c_double, ctypes.c_double, _PP, _PP,
                                                 _PP, ctypes.c_void_p]
     lib.dg_lserk_stage_adjoint_fp32.restype = ctypes.c_int
+    lib.dg_adjoint_space.argtypes = [ctypes.c_void_p, ctypes.c_int32, _PP, _PP, ctypes.c_void_p]
+    lib.dg_adjoint_space.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []
     lib.dg_last_error_string.restype = ctypes.c_char_p
     return lib

@@ -14,6 +14,7 @@ constexpr int kFaces = 4;
 constexpr int kGeo = 25;      // rx (9) + 4 faces x (nx, ny, nz, fscale)
 constexpr int kBlock = 256;
 constexpr int kMaxFields = 6;
+constexpr int kLdsPerCu = 160 * 1024;   // LDS bytes per CU (gfx950)
 
 constexpr int np_of(int order) { return (order + 1) * (order + 2) * (order + 3) / 6; }
 constexpr int nfp_of(int order) { return (order + 1) * (order + 2) / 2; }

@@ -308,7 +308,7 @@ void build_schedule(int64_t nelem, int unit, const double* centroids,
 template <typename T>
 using VjpKernel = void (*)(VjpArgs<T>);
 template <typename T>
-using MassInvKernel = void (*)(MassInvArgs<T>);
+using MassKernel = void (*)(MassArgs<T>);
 
 template <int EQ, typename T, int DIM>
 VjpKernel<T> pick_vjp_order(int order) {
@@ -321,12 +321,19 @@ VjpKernel<T> pick_vjp_order(int order) {
   }
 }
 
+// The FP64 adjoint of the linear operators runs on the forward kernel
+// (rhs_vjp_impl), so they have FP32 adjoint kernels only.
 template <typename T, int DIM>
 VjpKernel<T> pick_vjp_dim(int eq, int order) {
+  constexpr bool f32 = sizeof(T) == 4;
   switch (eq) {
-    case DG_EQ_WAVE: return pick_vjp_order<DG_EQ_WAVE, T, DIM>(order);
+    case DG_EQ_WAVE:
+      if constexpr (f32) return pick_vjp_order<DG_EQ_WAVE, T, DIM>(order);
+      return nullptr;
     case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER, T, DIM>(order);
-    case DG_EQ_MAXWELL: return pick_vjp_order<DG_EQ_MAXWELL, T, DIM>(order);
+    case DG_EQ_MAXWELL:
+      if constexpr (f32) return pick_vjp_order<DG_EQ_MAXWELL, T, DIM>(order);
+      return nullptr;
     default: return nullptr;
   }
 }
@@ -336,18 +343,29 @@ VjpKernel<T> pick_vjp(int eq, int order, int dim) {
   return dim == 2 ? pick_vjp_dim<T, 2>(eq, order) : pick_vjp_dim<T, 3>(eq, order);
 }
 
-// M^{-1} kernels by node count (tetrahedra: 4, 10, 20, 35; triangles: 3, 6,
-// 10, 15)
+// Mass-operator kernels by node count (tetrahedra: 4, 10, 20, 35; triangles:
+// 3, 6, 10, 15) and field count (3 to 6)
+template <typename T, int NP>
+MassKernel<T> mass_of_fields(int nf) {
+  switch (nf) {
+    case 3: return mass_apply_kernel<NP, 3, T>;
+    case 4: return mass_apply_kernel<NP, 4, T>;
+    case 5: return mass_apply_kernel<NP, 5, T>;
+    case 6: return mass_apply_kernel<NP, 6, T>;
+    default: return nullptr;
+  }
+}
+
 template <typename T>
-MassInvKernel<T> pick_mass_inv(int np) {
+MassKernel<T> pick_mass(int np, int nf) {
   switch (np) {
-    case 3: return mass_inv_kernel<3, T>;
-    case 4: return mass_inv_kernel<4, T>;
-    case 6: return mass_inv_kernel<6, T>;
-    case 10: return mass_inv_kernel<10, T>;
-    case 15: return mass_inv_kernel<15, T>;
-    case 20: return mass_inv_kernel<20, T>;
-    case 35: return mass_inv_kernel<35, T>;
+    case 3: return mass_of_fields<T, 3>(nf);
+    case 4: return mass_of_fields<T, 4>(nf);
+    case 6: return mass_of_fields<T, 6>(nf);
+    case 10: return mass_of_fields<T, 10>(nf);
+    case 15: return mass_of_fields<T, 15>(nf);
+    case 20: return mass_of_fields<T, 20>(nf);
+    case 35: return mass_of_fields<T, 35>(nf);
     default: return nullptr;
   }
 }
@@ -649,14 +667,16 @@ struct dg_plan {
   bool has_vjp = false;
   VjpKernel<double> vjp_kernel = nullptr;
   VjpKernel<float> vjp_kernel32 = nullptr;
-  MassInvKernel<double> massinv_kernel = nullptr;
-  MassInvKernel<float> massinv_kernel32 = nullptr;
+  MassKernel<double> mass_kernel = nullptr;
+  MassKernel<float> mass_kernel32 = nullptr;
   int vjp_grid = 0;
   int vjp_grid32 = 0;
-  int massinv_grid = 0;
+  int mass_grid = 0;
   double* amat_t = nullptr;      // (d, Np, Np): A_r = D_r^T M at [r][j][i]
   double* face_mass = nullptr;   // (F, Nfp, Nfp)
   double* mass_inv = nullptr;    // (Np, Np)
+  double* mass = nullptr;        // (Np, Np) reference mass M
+  double* jac = nullptr;         // (K) |J| per element
   double* ext_fscale = nullptr;  // (K, F)
   int32_t* node_faces = nullptr; // (Np, 3)
 };
@@ -686,6 +706,8 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->amat_t);
   (void)hipFree(p->face_mass);
   (void)hipFree(p->mass_inv);
+  (void)hipFree(p->mass);
+  (void)hipFree(p->jac);
   (void)hipFree(p->ext_fscale);
   (void)hipFree(p->node_faces);
   delete p;
@@ -840,7 +862,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       return fail(DG_ERR_INVALID_ARGUMENT, "face_nodes entry out of range");
   // adjoint data, built and checked before anything touches the device
   const bool want_vjp = d->mass_inv || d->face_mass;
-  std::vector<double> fm, xfs, amat_t;
+  std::vector<double> fm, xfs, amat_t, mass, jac;
   std::vector<int32_t> nf;
   if (want_vjp) {
     if (!d->mass_inv || !d->face_mass)
@@ -851,8 +873,20 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     if (!msg.empty()) return fail(DG_ERR_INVALID_ARGUMENT, msg);
     // A_r = D_r^T M, stored [r][j][i] = A_r[i][j]: the adjoint's volume
     // operator acting on mu = M^{-1} lam
-    std::vector<double> mass;
     if (!invert(np, d->mass_inv, mass)) return fail(DG_ERR_INVALID_ARGUMENT, "mass_inv is singular");
+    // |J| = 1 / |det(dr/dx)| per element, from the rx block of the geometry
+    // rows (the element mass matrix is |J| M)
+    jac.resize((size_t)k);
+    for (int64_t e = 0; e < k; ++e) {
+      const double* g = d->geometry + e * (tri ? kGeo2 : kGeo);
+      const double det = tri ? g[0] * g[3] - g[1] * g[2]
+                             : g[0] * (g[4] * g[8] - g[5] * g[7]) -
+                                   g[1] * (g[3] * g[8] - g[5] * g[6]) +
+                                   g[2] * (g[3] * g[7] - g[4] * g[6]);
+      if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+        return fail(DG_ERR_INVALID_ARGUMENT, "geometry: singular rx block");
+      jac[(size_t)e] = 1.0 / std::fabs(det);
+    }
     const double* dop = d->equation == DG_EQ_MAXWELL ? d->d_strong : d->d_weak;
     amat_t.assign((size_t)ndir * np * np, 0.0);
     for (int r = 0; r < ndir; ++r)
@@ -991,6 +1025,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     if ((rc = upload(&p->amat_t, amat_t.data(), amat_t.size())) ||
         (rc = upload(&p->face_mass, fm.data(), fm.size())) ||
         (rc = upload(&p->mass_inv, d->mass_inv, (size_t)np * np)) ||
+        (rc = upload(&p->mass, mass.data(), mass.size())) ||
+        (rc = upload(&p->jac, jac.data(), jac.size())) ||
         (rc = upload(&p->ext_fscale, xfs.data(), xfs.size())) ||
         (rc = upload(&p->node_faces, nf.data(), nf.size()))) {
       free_plan(p);
@@ -998,8 +1034,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     }
     p->vjp_kernel = pick_vjp<double>(d->equation, d->order, p->dim);
     p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order, p->dim);
-    p->massinv_kernel = pick_mass_inv<double>(np);
-    p->massinv_kernel32 = pick_mass_inv<float>(np);
+    p->mass_kernel = pick_mass<double>(np, p->nfields);
+    p->mass_kernel32 = pick_mass<float>(np, p->nfields);
     const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
     auto grid_of = [&](const void* fn) {
       int occ = 0;
@@ -1009,16 +1045,16 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       if (g > vbatch) g = vbatch;
       return (int)((g + 7) / 8 * 8);
     };
-    p->vjp_grid = grid_of((const void*)p->vjp_kernel);
+    if (p->vjp_kernel) p->vjp_grid = grid_of((const void*)p->vjp_kernel);
     p->vjp_grid32 = grid_of((const void*)p->vjp_kernel32);
     {
       int occ = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->massinv_kernel,
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->mass_kernel,
                                                         kBlock, 0) != hipSuccess ||
           occ < 1)
         occ = 1;
       int64_t mg = (int64_t)prop.multiProcessorCount * occ;
-      p->massinv_grid = (int)(mg < vbatch ? mg : vbatch);
+      p->mass_grid = (int)(mg < vbatch ? mg : vbatch);
     }
     p->has_vjp = true;
   }
@@ -1121,6 +1157,29 @@ int dg_lserk_stage(const dg_plan* p, double a, double b, double dt, double* cons
 
 namespace {
 
+// Signs of the field reflection R of the linear operators' adjoint: +1 on u
+// (wave), E (Maxwell) and E_z (TM), -1 on the velocity and H components.
+void adjoint_reflection(const dg_plan* p, double* sign) {
+  for (int c = 0; c < p->nfields; ++c) {
+    const bool plus = (p->equation == DG_EQ_MAXWELL && p->dim != 2) ? c < 3 : c == 0;
+    sign[c] = plus ? 1.0 : -1.0;
+  }
+}
+
+// Vector-Jacobian product.
+//
+// FP64 wave and Maxwell: with exact quadrature on affine elements the weak
+// form B = M L of these operators is a skew-symmetric central part plus a
+// symmetric upwind penalty, and the reflection R (the sign flip of the
+// velocity / magnetic field) negates exactly the central part, so
+// B^T = R B R and
+//     L^T lam = R M L R M^{-1} lam
+// (M the global mass matrix, |J_k| M_ref per element).  Three launches: the
+// scaled inverse mass into the workspace, the plan's forward RHS kernel into
+// q_bar, the scaled mass in place on q_bar.
+//
+// Euler (any precision) and the FP32 adjoint: mu = M_ref^{-1} lam into the
+// workspace, then the dedicated adjoint kernel.
 template <typename T>
 int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T* const* q_bar,
                  T* const* workspace, void* stream, const char* name) {
@@ -1135,29 +1194,54 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
       (rc = check_ptrs((const void* const*)q_bar, p->nfields, "q_bar")) ||
       (rc = check_ptrs((const void* const*)workspace, p->nfields, "workspace")))
     return rc;
-  const bool f32 = sizeof(T) == 4;
-  MassInvArgs<T> m{};
+  constexpr bool f32 = sizeof(T) == 4;
+  MassArgs<T> m{};
+  for (int c = 0; c < p->nfields; ++c) {
+    m.in[c] = rhs_bar[c];
+    m.out[c] = workspace[c];
+    m.sign[c] = 1.0;
+  }
+  m.matrix = p->mass_inv;
+  m.nelem = p->nelem;
+  MassKernel<T> mk;
+  if constexpr (f32)
+    mk = p->mass_kernel32;
+  else
+    mk = p->mass_kernel;
+
+  if constexpr (!f32) {
+    if (p->equation != DG_EQ_EULER) {
+      m.jac = p->jac;
+      m.jac_power = -1;
+      adjoint_reflection(p, m.sign);
+      hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+      DG_HIP_CHECK(hipGetLastError());
+      if ((rc = dg_rhs(p, DG_PREC_FP64, (const double* const*)workspace, q_bar, stream)) != DG_OK)
+        return rc;
+      for (int c = 0; c < p->nfields; ++c) {
+        m.in[c] = q_bar[c];
+        m.out[c] = q_bar[c];
+      }
+      m.matrix = p->mass;
+      m.jac_power = 1;
+      hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+      DG_HIP_CHECK(hipGetLastError());
+      return DG_OK;
+    }
+  }
+
   VjpArgs<T> a{};
   for (int c = 0; c < p->nfields; ++c) {
-    m.lam[c] = rhs_bar[c];
-    m.mu[c] = workspace[c];
     a.q[c] = q ? q[c] : nullptr;
     a.mu[c] = workspace[c];
     a.qbar[c] = q_bar[c];
   }
-  m.mass_inv = p->mass_inv;
-  m.nelem = p->nelem;
-  m.nf = p->nfields;
-  MassInvKernel<T> mk;
   VjpKernel<T> vk;
-  if constexpr (sizeof(T) == 4) {
-    mk = p->massinv_kernel32;
+  if constexpr (f32)
     vk = p->vjp_kernel32;
-  } else {
-    mk = p->massinv_kernel;
+  else
     vk = p->vjp_kernel;
-  }
-  hipLaunchKernelGGL(mk, dim3(p->massinv_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+  hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
   DG_HIP_CHECK(hipGetLastError());
   a.sched = {p->sched_batch, p->part_batch};
   a.geo = p->geo;
@@ -1213,6 +1297,32 @@ int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rh
 int dg_rhs_vjp_fp32(const dg_plan* p, const float* const* q, const float* const* rhs_bar,
                     float* const* q_bar, float* const* workspace, void* stream) {
   return rhs_vjp_impl<float>(p, q, rhs_bar, q_bar, workspace, stream, "dg_rhs_vjp_fp32");
+}
+
+int dg_adjoint_space(const dg_plan* p, int32_t to_adjoint, const double* const* in,
+                     double* const* out, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->has_vjp)
+    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: plan created without mass_inv / face_mass");
+  if (p->equation == DG_EQ_EULER)
+    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: the Euler operator is not linear");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
+    return rc;
+  MassArgs<double> m{};
+  for (int c = 0; c < p->nfields; ++c) {
+    m.in[c] = in[c];
+    m.out[c] = out[c];
+  }
+  adjoint_reflection(p, m.sign);
+  m.matrix = to_adjoint ? p->mass_inv : p->mass;
+  m.jac = p->jac;
+  m.jac_power = to_adjoint ? -1 : 1;
+  m.nelem = p->nelem;
+  hipLaunchKernelGGL(p->mass_kernel, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
 }
 
 int dg_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt, double* const* qbar,

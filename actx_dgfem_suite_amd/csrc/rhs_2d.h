@@ -183,12 +183,32 @@ __global__ __launch_bounds__(kBlock) void rhs_2d_kernel(Rhs2DArgs<T> a) {
 #pragma unroll
       for (int c = 0; c < NF; ++c) qx[r][c] = ext[r] >= 0 ? (double)a.q[c][ext[r]] : 0.0;
     __syncthreads();
+    {
+      // Every staging load in flight before the first LDS store (a
+      // load-then-store loop per field waits out one round trip per field);
+      // EB * NP <= kBlock, so a thread stages at most one node per field.
+      constexpr int RG = (EB * kGeo2 + kBlock - 1) / kBlock;
+      const bool own = tid < nel * NP;
+      double v[NF], vg[RG];
+      if (own) {
 #pragma unroll
-    for (int c = 0; c < NF; ++c) {
-      const T* src = a.q[c] + e0 * NP;
-      for (int k = tid; k < nel * NP; k += kBlock) s_q[c][k] = (double)src[k];
+        for (int c = 0; c < NF; ++c) v[c] = (double)a.q[c][e0 * NP + tid];
+      }
+#pragma unroll
+      for (int r = 0; r < RG; ++r) {
+        const int k = tid + r * kBlock;
+        if (k < nel * kGeo2) vg[r] = a.geo[e0 * kGeo2 + k];
+      }
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) s_q[c][tid] = v[c];
+      }
+#pragma unroll
+      for (int r = 0; r < RG; ++r) {
+        const int k = tid + r * kBlock;
+        if (k < nel * kGeo2) s_geo[k] = vg[r];
+      }
     }
-    for (int k = tid; k < nel * kGeo2; k += kBlock) s_geo[k] = a.geo[e0 * kGeo2 + k];
     __syncthreads();
 
     const bool active = el < nel;

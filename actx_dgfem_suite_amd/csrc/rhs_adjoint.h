@@ -54,13 +54,18 @@ struct VjpArgs {
   double p0;
 };
 
+// Per-element mass operator on every field: out_c = sign_c * J^p * A in_c,
+// with A the reference M^{-1} or M (symmetric), J the element Jacobian (p = -1
+// or +1; no scaling without a Jacobian table).  out may alias in.
 template <typename T>
-struct MassInvArgs {
-  const T* lam[kMaxFields];
-  T* mu[kMaxFields];
-  const double* mass_inv;        // (Np, Np), symmetric
+struct MassArgs {
+  const T* in[kMaxFields];
+  T* out[kMaxFields];
+  const double* matrix;          // (Np, Np), symmetric
+  const double* jac;             // (K) |J| per element, or NULL
+  int32_t jac_power;             // -1: divide by J, +1: multiply by J
+  double sign[kMaxFields];
   int64_t nelem;
-  int nf;
 };
 
 // {{{ flux VJPs: given psi (cotangent of the flux 'out' of FluxOf<EQ>::flux),
@@ -346,34 +351,56 @@ template <> struct Adj2DOf<DG_EQ_MAXWELL> { using type = Maxwell2DAdj; };
 
 // }}}
 
-// {{{ mu = M^{-1} lam per element: batches of kBlock / Np elements; every
+// {{{ per-element mass operator: batches of kBlock / Np elements; every
 // field of a batch is staged in LDS with coalesced loads, then one thread per
-// (element, node) forms its row of M^{-1} lam and stores it coalesced.  A
+// (element, node) forms its row of A in and stores it coalesced.  A
 // persistent grid of several blocks per CU keeps loads of many batches in
-// flight.
+// flight.  In place is safe: a block reads its whole batch into LDS (two
+// barriers) before any thread of it writes, and batches are disjoint.
 
-template <int NP, typename T>
-__global__ __launch_bounds__(kBlock) void mass_inv_kernel(MassInvArgs<T> a) {
+template <int NP, int NF, typename T>
+__global__ __launch_bounds__(kBlock) void mass_apply_kernel(MassArgs<T> a) {
   constexpr int EB = kBlock / NP;
-  __shared__ double s_m[NP * NP];     // s_m[j * NP + i] = Minv[i][j] (= Minv[j][i])
-  __shared__ double s_l[kMaxFields][EB * NP];
+  __shared__ double s_m[NP * NP];     // s_m[j * NP + i] = A[i][j] (= A[j][i])
+  __shared__ double s_l[NF][EB * NP];
   const int tid = threadIdx.x;
-  for (int k = tid; k < NP * NP; k += kBlock) s_m[k] = a.mass_inv[k];
+  for (int k = tid; k < NP * NP; k += kBlock) s_m[k] = a.matrix[k];
   const int el = tid / NP, i = tid - el * NP;
   const int64_t nbatch = (a.nelem + EB - 1) / EB;
+  // Each thread keeps its row of A in registers, so the contraction reads
+  // only the (broadcast) staged values from LDS: with A from LDS too the
+  // kernel is bound by LDS reads, two per multiply-add (measured on MI355X:
+  // wave P3 58 -> 35 us per pass).
+  double arow[NP];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NP; ++j) arow[j] = s_m[j * NP + i];
   for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
     const int64_t e0 = b * EB;
     const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
+    const bool own = tid < nel * NP;   // EB * NP <= kBlock: one node per thread
+    // all fields' loads in flight before the first LDS store
+    double v[NF];
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < NF; ++c) v[c] = (double)a.in[c][e0 * NP + tid];
+    }
+    double scale = 1.0;
+    if (a.jac && el < nel) scale = a.jac_power > 0 ? a.jac[e0 + el] : 1.0 / a.jac[e0 + el];
     __syncthreads();
-    for (int c = 0; c < a.nf; ++c)
-      for (int k = tid; k < nel * NP; k += kBlock) s_l[c][k] = (double)a.lam[c][e0 * NP + k];
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < NF; ++c) s_l[c][tid] = v[c];
+    }
     __syncthreads();
     if (el < nel) {
-      for (int c = 0; c < a.nf; ++c) {
+#pragma unroll
+      for (int c = 0; c < NF; ++c) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < NP; ++j) acc += s_m[j * NP + i] * s_l[c][el * NP + j];
-        a.mu[c][e0 * NP + tid] = (T)acc;
+        for (int j = 0; j < NP; ++j)
+          acc += arow[j] * s_l[c][el * NP + j];
+        a.out[c][e0 * NP + tid] = (T)(a.sign[c] * (scale * acc));
       }
     }
   }
@@ -436,6 +463,15 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
   __shared__ double s_x[NF][EB * NFN];
   __shared__ double s_geo[EB * GEO];
   __shared__ double s_xfs[EB * NFACES];
+  // the batch's exterior map, kept from the gathers where it costs no block
+  // per CU (else the face phase reads it again from global memory)
+  constexpr int LDS_BASE =
+      8 * (DIM * NP * NP + NFACES * NFP * NFP +
+           (NF + (NEEDS_Q ? NQ : 0) + (EQ == DG_EQ_WAVE ? NW : 0)) * EB * NP + 2 * NF * EB * NFN +
+           EB * GEO + EB * NFACES) +
+      4 * (NFN + 3 * NP);
+  constexpr bool MAP_LDS = kLdsPerCu / (LDS_BASE + 4 * EB * NFN) == kLdsPerCu / LDS_BASE;
+  __shared__ int32_t s_ext[MAP_LDS ? EB * NFN : 1];
 
   const int tid = threadIdx.x;
   for (int k = tid; k < DIM * NP * NP; k += kBlock) s_a[k] = a.amat_t[k];
@@ -453,6 +489,9 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
   // this batch is processed, so their latency overlaps the face and volume
   // phases.  R face-node slots per thread cover the batch's EB * NFN nodes.
   constexpr int R = (EB * NFN + kBlock - 1) / kBlock;
+  constexpr int RG = (EB * GEO + kBlock - 1) / kBlock;   // geometry values per thread
+  constexpr int GR = RG < 4 ? RG : 4;
+  static_assert(EB * NP <= kBlock && EB * NFACES <= kBlock, "one staged node per thread");
   int32_t pf_ext[R];
   double pf_mu[R][NF];
   double pf_q[R][NQ];
@@ -493,6 +532,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
     for (int r = 0; r < R; ++r) {
       const int k = tid + r * kBlock;
       if (k < nel * NFN) {
+        if constexpr (MAP_LDS) s_ext[k] = pf_ext[r];
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
           s_mux[c][k] = pf_mu[r][c];
@@ -502,14 +542,50 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
     }
     const bool has_next = pos + p_step < p_end;
     if (has_next) issue_map(a.sched.order[pos + p_step]);
+    // The staging loads of a round are issued together before their LDS
+    // stores (a load-then-store loop per field would wait out one memory
+    // round trip per field): mu and Fscale, then the geometry, then q
+    // (Euler), so that the registers of the rounds do not add up.
+    // EB * NP <= kBlock and EB * NFACES <= kBlock, so each thread stages at
+    // most one node of each field and one face's Fscale.
+    const bool own = tid < nel * NP;
+    {
+      double v_mu[NF], v_xfs = 0.0;
+      if (own) {
 #pragma unroll
-    for (int c = 0; c < NF; ++c)
-      for (int k = tid; k < nel * NP; k += kBlock) {
-        s_mu[c][k] = (double)a.mu[c][e0 * NP + k];
-        if constexpr (NEEDS_Q) s_q[c][k] = (double)a.q[c][e0 * NP + k];
+        for (int c = 0; c < NF; ++c) v_mu[c] = (double)a.mu[c][e0 * NP + tid];
       }
-    for (int k = tid; k < nel * GEO; k += kBlock) s_geo[k] = a.geo[e0 * GEO + k];
-    for (int k = tid; k < nel * NFACES; k += kBlock) s_xfs[k] = a.ext_fscale[e0 * NFACES + k];
+      if (tid < nel * NFACES) v_xfs = a.ext_fscale[e0 * NFACES + tid];
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) s_mu[c][tid] = v_mu[c];
+      }
+      if (tid < nel * NFACES) s_xfs[tid] = v_xfs;
+    }
+    // geometry: at most GR values per thread in flight per round
+#pragma unroll 1
+    for (int r0 = 0; r0 < RG; r0 += GR) {
+      double v_geo[GR];
+#pragma unroll
+      for (int r = 0; r < GR; ++r) {
+        const int k = tid + (r0 + r) * kBlock;
+        if (r0 + r < RG && k < nel * GEO) v_geo[r] = a.geo[e0 * GEO + k];
+      }
+#pragma unroll
+      for (int r = 0; r < GR; ++r) {
+        const int k = tid + (r0 + r) * kBlock;
+        if (r0 + r < RG && k < nel * GEO) s_geo[k] = v_geo[r];
+      }
+    }
+    if constexpr (NEEDS_Q) {
+      double v_q[NQ];
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) v_q[c] = (double)a.q[c][e0 * NP + tid];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) s_q[c][tid] = v_q[c];
+      }
+    }
     if (has_next) issue_gathers();
     __syncthreads();
 
@@ -553,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
         psi[c] = fs * so;
         psin[c] = xfs * sn;
       }
-      const int32_t ext = a.emap[(e0 + fe) * NFN + fn];
+      const int32_t ext = MAP_LDS ? s_ext[k] : a.emap[(e0 + fe) * NFN + fn];
       double qm[NF], qp[NF], gM[NF], gP[NF];
 #pragma unroll
       for (int c = 0; c < NF; ++c) {

@@ -78,12 +78,32 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
     const int nel = batch_nel(b);
     const bool has_next = pos + p_step < p_end;
     __syncthreads();
+    {
+      // Every staging load in flight before the first LDS store (a
+      // load-then-store loop per field waits out one round trip per field);
+      // EB * NP <= kBlock, so a thread stages at most one node per field.
+      constexpr int RG = (EB * kGeo + kBlock - 1) / kBlock;
+      const bool own = tid < nel * NP;
+      double v[NF], vg[RG];
+      if (own) {
 #pragma unroll
-    for (int c = 0; c < NF; ++c) {
-      const double* src = a.q.p[c] + e0 * NP;
-      for (int k = tid; k < nel * NP; k += kBlock) s_q[c][k] = src[k];
+        for (int c = 0; c < NF; ++c) v[c] = a.q.p[c][e0 * NP + tid];
+      }
+#pragma unroll
+      for (int r = 0; r < RG; ++r) {
+        const int k = tid + r * kBlock;
+        if (k < nel * kGeo) vg[r] = a.geo[e0 * kGeo + k];
+      }
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) s_q[c][tid] = v[c];
+      }
+#pragma unroll
+      for (int r = 0; r < RG; ++r) {
+        const int k = tid + r * kBlock;
+        if (k < nel * kGeo) s_geo[k] = vg[r];
+      }
     }
-    for (int k = tid; k < nel * kGeo; k += kBlock) s_geo[k] = a.geo[e0 * kGeo + k];
     __syncthreads();
 
     const bool active = el < nel;

@@ -262,6 +262,25 @@ class DGPlan:
             None if w is None else _lib.pointer_array([t.data_ptr() for t in w]),
             ctypes.c_void_p(stream)), name)
 
+    def adjoint_space(self, fields: Sequence[torch.Tensor], to_adjoint: bool,
+                      out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:
+        """``R M^{-1} fields`` (``to_adjoint``) or ``R M fields``: the change of
+        variables in which the linear operators' transpose is the forward RHS
+        (``dg_adjoint_space``; float64 wave and Maxwell plans).  ``out`` may be
+        ``fields`` itself."""
+        self._check_fields(fields, "fields", torch.float64)
+        if out is None:
+            out = self.empty_fields()
+        else:
+            self._check_fields(out, "out", torch.float64)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_adjoint_space(
+            self._handle, 1 if to_adjoint else 0,
+            _lib.pointer_array([t.data_ptr() for t in fields]),
+            _lib.pointer_array([t.data_ptr() for t in out]),
+            ctypes.c_void_p(stream)), "dg_adjoint_space")
+        return list(out)
+
     def close(self) -> None:
         h = getattr(self, "_handle", None)
         if h is not None and h.value:

@@ -117,12 +117,38 @@ class LSERK4:
             buf = self._adj[dtype] = (self.plan.empty_fields(dtype), self.plan.empty_fields(dtype))
         return buf
 
+    def _adjoint_space_sweep(self, q_hat: Sequence[torch.Tensor]) -> bool:
+        """Whether a float64 cotangent of this stepper is swept in the
+        variables of ``plan.adjoint_space`` (linear operators only)."""
+        return self.linear and q_hat[0].dtype == torch.float64
+
+    def step_vjp_adjoint_space(self, q_hat: Sequence[torch.Tensor]) -> None:
+        """One reverse step of a linear operator on a float64 cotangent held
+        in the variables ``q_hat = R M^{-1} q_bar`` of ``plan.adjoint_space``.
+        There the transposed RHS is the forward RHS (``L^T = R M L R
+        M^{-1}``), so each reverse stage is one ``dg_rhs`` and one
+        ``dg_lserk_stage_adjoint``: the reverse step costs what a forward
+        step does."""
+        plan = self.plan
+        s, w = self._adjoint_buffers(torch.float64)
+        plan.lserk_stage_adjoint(0.0, LSERK4_B[4], self.dt, q_hat, s, None)
+        for k in range(4, -1, -1):
+            plan.rhs(s, w)
+            a = LSERK4_A[k]
+            b = LSERK4_B[k - 1] if k > 0 else 0.0
+            plan.lserk_stage_adjoint(a, b, self.dt, q_hat, s, w)
+
     def step_vjp(self, q_n: Sequence[torch.Tensor] | None, q_bar: Sequence[torch.Tensor]) -> None:
         """Replace ``q_bar`` (the cotangent of q^{n+1}) in place by that of
         q^n.  ``q_n`` is the FP64 state at the start of the step (left
         intact); linear operators do not read it and accept ``None``."""
         plan = self.plan
         dtype = q_bar[0].dtype
+        if self._adjoint_space_sweep(q_bar):
+            plan.adjoint_space(q_bar, True, out=q_bar)
+            self.step_vjp_adjoint_space(q_bar)
+            plan.adjoint_space(q_bar, False, out=q_bar)
+            return
         s, w = self._adjoint_buffers(dtype)
         stages: list[list[torch.Tensor]] | None = None
         if not self.linear:
@@ -171,6 +197,13 @@ def reverse_sweep(stepper: LSERK4, checkpoints: dict[int, list[torch.Tensor]], n
     lam = _clone(q_bar_final)
     if stepper.linear:
         checkpoints.clear()
+        if stepper._adjoint_space_sweep(lam):
+            # the whole sweep in the adjoint variables: one transform each way
+            stepper.plan.adjoint_space(lam, True, out=lam)
+            for _ in range(nsteps):
+                stepper.step_vjp_adjoint_space(lam)
+            stepper.plan.adjoint_space(lam, False, out=lam)
+            return lam
         for _ in range(nsteps):
             stepper.step_vjp(None, lam)
         return lam

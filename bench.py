@@ -233,8 +233,8 @@ def main():
 
     adjoint = None
     if not mixed and args.adjoint_steps > 0:
-        # VJP of the same RHS (the M^{-1} pass and the adjoint kernel), timed
-        # the same way; cotangent = the forward output
+        # VJP of the same RHS, timed the same way (all launches of one
+        # dg_rhs_vjp); cotangent = the forward output
         lam = [t.clone() for t in out]
         qv = q if w.equation == "euler" else None
         qbar = plan.empty_fields()
@@ -251,7 +251,9 @@ def main():
         vjp_ms = sum(a.elapsed_time(b) for a, b in zip(va, vb)) / args.adjoint_steps
         adjoint = {"vjp_us": vjp_ms * 1e3, "fwd_us": kernel_ms * 1e3,
                    "ratio": vjp_ms / kernel_ms, "steps": args.adjoint_steps,
-                   "kernels": "mass_inv_kernel + rhs_vjp_kernel"}
+                   "kernels": ("mass_apply_kernel + rhs_vjp_kernel" if w.equation == "euler"
+                               else "mass_apply_kernel (R M^-1) + forward RHS kernel "
+                                    "+ mass_apply_kernel (R M)")}
         del lam, qbar
 
     if args.copy_probe:

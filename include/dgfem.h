@@ -164,8 +164,11 @@ int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
  *   q_bar = (dL/dq)^T rhs_bar
  * the exact transpose of dg_rhs for the linear wave and Maxwell operators
  * (q may then be NULL) and of dg_rhs linearised at q for Euler.  `workspace`
- * is nfields device buffers of (K, Np) float64 (overwritten; holds
- * M^{-1} rhs_bar).  Two kernels are launched on `stream`.  The reference has
+ * is nfields device buffers of (K, Np) float64 (overwritten).  Wave and
+ * Maxwell run three kernels on `stream` -- R M^{-1} rhs_bar into the
+ * workspace, the dg_rhs kernel into q_bar, R M in place (the identity
+ * L^T = R M L R M^{-1} of their upwind weak forms, dg_adjoint_space) --
+ * Euler two: M_ref^{-1} rhs_bar, then the adjoint kernel.  The reference has
  * no adjoint: its jax:jit array context (run/__main__.py:213) is where
  * jax.vjp of the same RHS would come from.  Needs a plan created with
  * mass_inv and face_mass, else DG_ERR_UNSUPPORTED. */
@@ -195,6 +198,18 @@ int dg_lserk_stage_adjoint(const dg_plan* plan, double a, double b, double dt,
 int dg_lserk_stage_adjoint_fp32(const dg_plan* plan, double a, double b, double dt,
                                 float* const* q_bar, float* const* s,
                                 const float* const* w, void* stream);
+
+/* Change of variables of the linear operators' adjoint (wave, Maxwell; FP64):
+ *   to_adjoint != 0:  out = R M^{-1} in;     to_adjoint == 0:  out = R M in
+ * with M the global mass matrix (|J| M_ref per element) and R the sign flip
+ * of the velocity (wave) or magnetic (Maxwell) fields; in and out may be the
+ * same buffers.  In these variables the transpose of the RHS is the RHS
+ * itself -- L^T = R M L R M^{-1}, see dg_rhs_vjp -- so a reverse LSERK4 sweep
+ * runs on dg_rhs and dg_lserk_stage_adjoint between one transform in and one
+ * out.  Needs a plan created with mass_inv and face_mass; Euler plans return
+ * DG_ERR_UNSUPPORTED.  No counterpart in the reference (see dg_rhs_vjp). */
+int dg_adjoint_space(const dg_plan* plan, int32_t to_adjoint, const double* const* in,
+                     double* const* out, void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* dg_last_error_string(void);

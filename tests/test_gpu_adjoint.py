@@ -164,6 +164,36 @@ def test_euler_vjp_central_difference(order, device):
     assert rel < 1e-8
 
 
+@pytest.mark.parametrize("equation", ["wave", "maxwell"])
+def test_adjoint_space_against_the_oracle_mass(equation, device):
+    """dg_adjoint_space: R M^{-1} and R M with M = |J| M_ref per element from
+    the oracle's reference mass matrix and Jacobians (R: +1 on u / E, -1 on
+    v / H), in place and out of place; the two directions invert each other."""
+    print(f"[case] adjoint space {equation} P2 n=3", flush=True)
+    d, od = discr_pair(2, 3)
+    plan = DGPlan(d, equation, PARAM[equation])
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((NF[equation], d.nelements, d.np))
+    sign = np.array([1.0] + [-1.0] * 3 if equation == "wave" else [1.0] * 3 + [-1.0] * 3)
+    mref, jac = od.ref.mass, np.abs(od.jac)
+    want_inv = sign[:, None, None] * np.einsum("ij,cej->cei", np.linalg.inv(mref), x) / jac[None, :, None]
+    want_m = sign[:, None, None] * np.einsum("ij,cej->cei", mref, x) * jac[None, :, None]
+    got_inv = _np(plan.adjoint_space(_dev(x, device), True))
+    xd = _dev(x, device)
+    got_m = _np(plan.adjoint_space(xd, False, out=xd))       # in place
+    assert np.abs(got_inv - want_inv).max() <= 1e-13 * np.abs(want_inv).max()
+    assert np.abs(got_m - want_m).max() <= 1e-13 * np.abs(want_m).max()
+    back = _np(plan.adjoint_space(plan.adjoint_space(_dev(x, device), True), False))
+    assert np.abs(back - x).max() <= 1e-13 * np.abs(x).max()
+
+
+def test_adjoint_space_rejects_euler(device):
+    d, _ = discr_pair(1, 2)
+    plan = DGPlan(d, "euler", PARAM["euler"])
+    with pytest.raises(NotImplementedError, match="not linear"):
+        plan.adjoint_space(plan.empty_fields(), True)
+
+
 def test_vjp_many_batches_and_workspace_reuse(device):
     """A mesh where every persistent block runs several batches; two calls in
     a row with different cotangents reuse the workspace."""
