@@ -321,18 +321,18 @@ VjpKernel<T> pick_vjp_order(int order) {
   }
 }
 
-// The FP64 adjoint of the linear operators runs on the forward kernel
-// (rhs_vjp_impl), so they have FP32 adjoint kernels only.
+// The FP64 adjoint of the linear operators on tetrahedra runs on the forward
+// kernel (rhs_vjp_impl), so there they have FP32 adjoint kernels only.
 template <typename T, int DIM>
 VjpKernel<T> pick_vjp_dim(int eq, int order) {
-  constexpr bool f32 = sizeof(T) == 4;
+  constexpr bool kernel_for_linear = sizeof(T) == 4 || DIM == 2;
   switch (eq) {
     case DG_EQ_WAVE:
-      if constexpr (f32) return pick_vjp_order<DG_EQ_WAVE, T, DIM>(order);
+      if constexpr (kernel_for_linear) return pick_vjp_order<DG_EQ_WAVE, T, DIM>(order);
       return nullptr;
     case DG_EQ_EULER: return pick_vjp_order<DG_EQ_EULER, T, DIM>(order);
     case DG_EQ_MAXWELL:
-      if constexpr (f32) return pick_vjp_order<DG_EQ_MAXWELL, T, DIM>(order);
+      if constexpr (kernel_for_linear) return pick_vjp_order<DG_EQ_MAXWELL, T, DIM>(order);
       return nullptr;
     default: return nullptr;
   }
@@ -1168,18 +1168,20 @@ void adjoint_reflection(const dg_plan* p, double* sign) {
 
 // Vector-Jacobian product.
 //
-// FP64 wave and Maxwell: with exact quadrature on affine elements the weak
-// form B = M L of these operators is a skew-symmetric central part plus a
-// symmetric upwind penalty, and the reflection R (the sign flip of the
-// velocity / magnetic field) negates exactly the central part, so
-// B^T = R B R and
+// FP64 wave and Maxwell on tetrahedra: with exact quadrature on affine
+// elements the weak form B = M L of these operators is a skew-symmetric
+// central part plus a symmetric upwind penalty, and the reflection R (the
+// sign flip of the velocity / magnetic field) negates exactly the central
+// part, so B^T = R B R and
 //     L^T lam = R M L R M^{-1} lam
 // (M the global mass matrix, |J_k| M_ref per element).  Three launches: the
 // scaled inverse mass into the workspace, the plan's forward RHS kernel into
-// q_bar, the scaled mass in place on q_bar.
+// q_bar, the scaled mass in place on q_bar.  Measured on MI355X: wave P3
+// 259 -> 209 us, wave P4 366 -> 277 us, Maxwell P4 711 -> 516 us.
 //
-// Euler (any precision) and the FP32 adjoint: mu = M_ref^{-1} lam into the
-// workspace, then the dedicated adjoint kernel.
+// Euler, triangles and the FP32 adjoint: mu = M_ref^{-1} lam into the
+// workspace, then the dedicated adjoint kernel (on triangles the two mass
+// passes cost more than the adjoint kernel: wave P1 177 against 159 us).
 template <typename T>
 int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T* const* q_bar,
                  T* const* workspace, void* stream, const char* name) {
@@ -1210,7 +1212,7 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
     mk = p->mass_kernel;
 
   if constexpr (!f32) {
-    if (p->equation != DG_EQ_EULER) {
+    if (p->equation != DG_EQ_EULER && p->dim != 2) {
       m.jac = p->jac;
       m.jac_power = -1;
       adjoint_reflection(p, m.sign);
