@@ -782,6 +782,9 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
   a.geo = p->geo;
   a.emap = p->emap;
   a.fmask = p->fmask;
+  a.vgeo = p->vgeo;
+  a.fgeo = p->fgeo;
+  a.ext_t = p->ext_t;
   a.op_tiles = p->mixed_op[k];
   a.lift_tiles = p->mixed_lift[k];
   a.nelem = p->nelem;
@@ -942,11 +945,12 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       return rc;
     }
   }
-  if (!tri && p->variant == DG_KERNEL_MFMA) {
+  if (!tri) {
     // tile-ordered geometry and flux-item-ordered exterior map of the FP64
-    // MFMA kernel (the same data as geometry / ext_map, laid out so that each
-    // of the kernel's loads is contiguous; measured on MI355X: wave P3 -10%,
-    // Maxwell P4 -6%, identical results)
+    // MFMA kernel, which the mixed kernels read too (the same data as
+    // geometry / ext_map, laid out so that each of the kernels' loads is
+    // contiguous; measured on MI355X: FP64 wave P3 -10%, Maxwell P4 -6%,
+    // identical results)
     std::vector<double> vgeo, fgeo;
     std::vector<int32_t> ext_t;
     build_tile_geometry(k, d->geometry, vgeo, fgeo);

@@ -25,6 +25,7 @@
 #pragma once
 
 #include "dg_common.h"
+#include "rhs_mfma.h"
 
 namespace dg {
 
@@ -38,6 +39,11 @@ struct MixedArgs {
   const double* geo;        // (K, 25), FP64
   const int32_t* emap;      // (K, 4*Nfp)
   const int32_t* fmask;     // (4*Nfp)
+  // the FP64 MFMA kernel's tile-ordered tables (rhs_mfma.h): rx, face
+  // geometry, and the exterior map in that kernel's flux-item order
+  const double* vgeo;       // [tile][9][16]
+  const double* fgeo;       // [tile][face*16 + n][4]
+  const int32_t* ext_t;     // [tile][chunk][item][64]
   const void* op_tiles;     // A tiles of D (float, or bf16 hi/lo pairs)
   const void* lift_tiles;   // A tiles of LIFT
   int64_t nelem;
@@ -142,6 +148,24 @@ constexpr bool mixed_branch_free_gathers() {
   return !(EQ == DG_EQ_MAXWELL && ORDER >= 3);
 }
 
+// Whether the kernel reads rx, the face geometry and the exterior map from
+// the FP64 MFMA kernel's tile-ordered tables, producing the flux in that
+// kernel's chunk order (one contiguous load per slot), instead of from the
+// element-ordered geometry and map.  Measured on MI355X, bit-identical
+// results: FP32 wave P1 -11%, P2 -3%, Euler P4 -5%, Maxwell P1 -12%, P2 -3%;
+// BF16x3 wave P2 -5%, P3 -11%, Euler P2-P4 -3% to -11%, Maxwell P1 -12%,
+// P2 -19%.  Off where it was even or slower: FP32 Euler P3 +61%, BF16x3
+// Maxwell P3 +15%, P4 +19%, and the rest within +-2% (FP32 wave P3-P4,
+// Euler P1-P2, Maxwell P3-P4; BF16x3 wave P1, P4 +4-5%, Euler P1).
+template <int EQ, int PREC, int ORDER>
+constexpr bool mixed_tile_tables() {
+  if constexpr (PREC == DG_PREC_FP32)
+    return (EQ != DG_EQ_EULER && ORDER <= 2) || (EQ == DG_EQ_EULER && ORDER == 4);
+  if constexpr (EQ == DG_EQ_WAVE) return ORDER == 2 || ORDER == 3;
+  if constexpr (EQ == DG_EQ_EULER) return ORDER >= 2;
+  return ORDER <= 2;
+}
+
 template <int EQ, int PREC, int ORDER>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   using S = MixedShape<PREC, ORDER>;
@@ -194,10 +218,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
-    const double* geo = a.geo + el * kGeo;
+    constexpr bool TT = mixed_tile_tables<EQ, PREC, ORDER>();
     double rx[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) rx[k] = geo[k];
+    for (int k = 0; k < 9; ++k) rx[k] = TT ? a.vgeo[(tile * 9 + k) * 16 + n] : a.geo[el * kGeo + k];
 
     f32x4 acc[NF][MB];
 #pragma unroll
@@ -367,26 +391,55 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 
     if constexpr (ST) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // staged state complete
 
+    // With the tile tables the items run in the FP64 MFMA kernel's chunk
+    // order (C LIFT k-steps per chunk; C divides Nfp), so the face geometry
+    // and the exterior map come with one contiguous load per slot; without,
+    // all KSL k-steps are one chunk and the element-ordered tables are read.
+    // BF16x3 pads the k-steps to KSL; the padding is zero.
+    constexpr int C = TT ? flux_chunk_of(ORDER) : KSL;
+    constexpr int NCH = TT ? NFP / C : 1;
+    static_assert(!TT || NFP % C == 0, "whole flux chunks");
+    if constexpr (TT) {
 #pragma unroll
-    for (int it = 0; it < KSL; ++it) {
+      for (int kk = NFP; kk < KSL; ++kk)
+#pragma unroll
+        for (int c = 0; c < NF; ++c) buf[(kk * NF + c) * FROW + lane] = 0.f;
+    }
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+    for (int it = 0; it < C; ++it) {
       const int item = it * 64 + lane;
-      const int kk = item % KSL;
-      const int p = item / KSL;                   // p = face * 16 + element
+      const int kk = ch * C + item % C;
+      const int p = item / C;                     // p = face * 16 + element
       const int fg = p >> 4, fe = p & 15;
       float outf[NF];
       if (kk < NFP) {
         int64_t pe = tile * 16 + fe;
         pe = pe < a.nelem ? pe : a.nelem - 1;
-        const double* pgeo = a.geo + pe * kGeo + 9 + 4 * fg;
-        const double nrm[3] = {pgeo[0], pgeo[1], pgeo[2]};
-        const double fscale = pgeo[3];
+        double nrm[3], fscale;
+        if constexpr (TT) {
+          const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + p) * 4);
+          const double2 g01 = pgeo[0], g23 = pgeo[1];
+          nrm[0] = g01.x;
+          nrm[1] = g01.y;
+          nrm[2] = g23.x;
+          fscale = g23.y;
+        } else {
+          const double* pgeo = a.geo + pe * kGeo + 9 + 4 * fg;
+          nrm[0] = pgeo[0];
+          nrm[1] = pgeo[1];
+          nrm[2] = pgeo[2];
+          fscale = pgeo[3];
+        }
         const int fn = fg * NFP + kk;
         const int64_t vn = pe * NP + s_fmask[fn];
         double qm[NF], qp[NF], out[NF];
 #pragma unroll
         for (int c = 0; c < NF; ++c)
           qm[c] = ST ? (double)stg[c * 16 * NP + fe * NP + s_fmask[fn]] : (double)a.q[c][vn];
-        const int32_t ext = a.emap[pe * NFN + fn];
+        const int32_t ext = TT ? a.ext_t[tile * flux_items_of(ORDER) + (ch * C + it) * 64 + lane]
+                               : a.emap[pe * NFN + fn];
         if constexpr (mixed_branch_free_gathers<EQ, PREC, ORDER>()) {
           const int64_t src = ext >= 0 ? (int64_t)ext : vn;
 #pragma unroll
