@@ -2,9 +2,13 @@
 //   q_bar = (dL/dq)^T lam   for rhs = L(q)
 // Exact transpose of the forward kernels' arithmetic for wave and Maxwell
 // (linear), and of the Euler operator linearised at q (flux Jacobians
-// recomputed per node).
+// recomputed per node).  This file holds the per-element mass operator
+// (mass_apply_kernel) and the adjoint kernel (rhs_vjp_kernel).  The FP64
+// adjoint of wave and Maxwell on tetrahedra does not use the adjoint kernel:
+// the host runs it as R M L R M^{-1} lam on the forward kernel (dgfem.hip,
+// rhs_vjp_impl).
 //
-// Two passes.  The first forms mu = M^{-1} lam per element.  Everything the
+// The adjoint kernel path has two passes.  The first forms mu = M^{-1} lam per element.  Everything the
 // second pass needs is a function of mu:
 //   volume:  D_r^T lam = (D_r^T M) mu = A_r mu       (A_r = S_r in weak form)
 //   surface: the forward term rhs_c[e,i] += sum_fn LIFT[i][fn] fs_e(f) Phi_c
