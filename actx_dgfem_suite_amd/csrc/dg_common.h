@@ -52,6 +52,11 @@ struct RhsArgs {
   const double* vgeo;        // [tile][9][16]: rx of element n of the tile
   const double* fgeo;        // [tile][face*16 + n][4]: n_x, n_y, n_z, Fscale
   const int32_t* ext_t;      // [tile][chunk][item][64]: emap in flux-item order
+  // MFMA kernel with the mass post-multiplication (the linear operators'
+  // adjoint): out_c = post_sign[c] * |J| * M_ref * rhs_c per element
+  const double* post_tiles;  // MFMA A tiles of M_ref: [mblock][kstep][64]
+  const double* jac;         // (K) |J| per element
+  double post_sign[kMaxFields];
   int64_t nelem;
   double p0;
 };

@@ -156,6 +156,26 @@ KernelInfo mfma_info() {
 DG_DEFINE_PICK(pick_lds, lds_info)
 DG_DEFINE_PICK(pick_mfma, mfma_info)
 
+// The MFMA kernel with the fused mass post-multiplication, for the adjoint of
+// the linear operators (wave, Maxwell).
+template <int EQ>
+RhsKernel pick_mfma_post_order(int order) {
+  switch (order) {
+    case 1: return rhs_mfma_kernel<EQ, 1, true>;
+    case 2: return rhs_mfma_kernel<EQ, 2, true>;
+    case 3: return rhs_mfma_kernel<EQ, 3, true>;
+    case 4: return rhs_mfma_kernel<EQ, 4, true>;
+    default: return nullptr;
+  }
+}
+RhsKernel pick_mfma_post(int eq, int order) {
+  switch (eq) {
+    case DG_EQ_WAVE: return pick_mfma_post_order<DG_EQ_WAVE>(order);
+    case DG_EQ_MAXWELL: return pick_mfma_post_order<DG_EQ_MAXWELL>(order);
+    default: return nullptr;
+  }
+}
+
 template <typename T>
 using TriKernel = void (*)(Rhs2DArgs<T>);
 
@@ -584,6 +604,21 @@ void build_tiles(int np, int nfp, const double* op, const double* lift,
       }
 }
 
+// MFMA A tiles of one (Np, Np) matrix in the layout of the D tiles above
+// (a short last block on the 4x4x4 form).
+std::vector<double> square_tiles(int np, const double* mat) {
+  const int mb = (np + 15) / 16, ks_n = (np + 3) / 4;
+  std::vector<double> t((size_t)mb * ks_n * 64, 0.0);
+  for (int b = 0; b < mb; ++b)
+    for (int ks = 0; ks < ks_n; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int m = 16 * b + ((b == mb - 1 && short_last_block(np)) ? (l & 3) : (l & 15));
+        const int j = (l >> 4) * ks_n + ks;
+        if (m < np && j < np) t[((size_t)b * ks_n + ks) * 64 + l] = mat[(size_t)m * np + j];
+      }
+  return t;
+}
+
 // Tile-ordered copies of the geometry for the MFMA kernels (16-element
 // tiles; slots past the last element repeat it):
 //   vgeo[t][x][n]        = geo[16 t + n][x]                 (x < 9: rx)
@@ -669,6 +704,9 @@ struct dg_plan {
   VjpKernel<float> vjp_kernel32 = nullptr;
   MassKernel<double> mass_kernel = nullptr;
   MassKernel<float> mass_kernel32 = nullptr;
+  RhsKernel post_kernel = nullptr;   // tetrahedral wave / Maxwell: fused M post-multiply
+  int post_grid = 0;
+  double* post_tiles = nullptr;      // A tiles of M_ref
   int vjp_grid = 0;
   int vjp_grid32 = 0;
   int mass_grid = 0;
@@ -708,6 +746,7 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->mass_inv);
   (void)hipFree(p->mass);
   (void)hipFree(p->jac);
+  (void)hipFree(p->post_tiles);
   (void)hipFree(p->ext_fscale);
   (void)hipFree(p->node_faces);
   delete p;
@@ -742,6 +781,30 @@ int check_ptrs(const void* const* ptrs, int n, const char* what) {
   for (int k = 0; k < n; ++k)
     if (!ptrs[k]) return fail(DG_ERR_INVALID_ARGUMENT, std::string(what) + "[" + std::to_string(k) + "] is NULL");
   return DG_OK;
+}
+
+// Arguments of the tetrahedral FP64 RHS kernels (batch schedule for the
+// LDS-staged kernel, tile schedule for the MFMA kernels).
+RhsArgs rhs_args(const dg_plan* p, const double* const* in, double* const* out, bool batches) {
+  RhsArgs a{};
+  for (int c = 0; c < p->nfields; ++c) {
+    a.q.p[c] = in[c];
+    a.rhs.p[c] = out[c];
+  }
+  a.sched = batches ? Schedule{p->sched_batch, p->part_batch} : Schedule{p->sched_tile, p->part_tile};
+  a.geo = p->geo;
+  a.emap = p->emap;
+  a.dmat_t = p->dmat_t;
+  a.lift_t = p->lift_t;
+  a.fmask = p->fmask;
+  a.op_tiles = p->op_tiles;
+  a.lift_tiles = p->lift_tiles;
+  a.vgeo = p->vgeo;
+  a.fgeo = p->fgeo;
+  a.ext_t = p->ext_t;
+  a.nelem = p->nelem;
+  a.p0 = p->params[0];
+  return a;
 }
 
 }  // namespace
@@ -1036,6 +1099,23 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       free_plan(p);
       return rc;
     }
+    if (!tri && d->equation != DG_EQ_EULER && p->variant == DG_KERNEL_MFMA) {
+      const std::vector<double> mt = square_tiles(np, mass.data());
+      if ((rc = upload(&p->post_tiles, mt.data(), mt.size()))) {
+        free_plan(p);
+        return rc;
+      }
+      p->post_kernel = pick_mfma_post(d->equation, d->order);
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->post_kernel, kBlock,
+                                                        0) != hipSuccess ||
+          occ < 1)
+        occ = 1;
+      int64_t pg = (int64_t)prop.multiProcessorCount * occ;
+      const int64_t tbatch = (k + 16 * (kBlock / 64) - 1) / (16 * (kBlock / 64));
+      if (pg > tbatch) pg = tbatch;
+      p->post_grid = (int)((pg + 7) / 8 * 8);
+    }
     p->vjp_kernel = pick_vjp<double>(d->equation, d->order, p->dim);
     p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order, p->dim);
     p->mass_kernel = pick_mass<double>(np, p->nfields);
@@ -1106,25 +1186,7 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
     DG_HIP_CHECK(hipGetLastError());
     return DG_OK;
   }
-  RhsArgs a{};
-  for (int c = 0; c < p->nfields; ++c) {
-    a.q.p[c] = in[c];
-    a.rhs.p[c] = out[c];
-  }
-  a.sched = p->variant == DG_KERNEL_LDS ? Schedule{p->sched_batch, p->part_batch}
-                                        : Schedule{p->sched_tile, p->part_tile};
-  a.geo = p->geo;
-  a.emap = p->emap;
-  a.dmat_t = p->dmat_t;
-  a.lift_t = p->lift_t;
-  a.fmask = p->fmask;
-  a.op_tiles = p->op_tiles;
-  a.lift_tiles = p->lift_tiles;
-  a.vgeo = p->vgeo;
-  a.fgeo = p->fgeo;
-  a.ext_t = p->ext_t;
-  a.nelem = p->nelem;
-  a.p0 = p->params[0];
+  const RhsArgs a = rhs_args(p, in, out, p->variant == DG_KERNEL_LDS);
   hipLaunchKernelGGL(p->kernel.fn, dim3(p->grid), dim3(p->kernel.threads), 0,
                      (hipStream_t)stream, a);
   DG_HIP_CHECK(hipGetLastError());
@@ -1178,10 +1240,13 @@ void adjoint_reflection(const dg_plan* p, double* sign) {
 // sign flip of the velocity / magnetic field) negates exactly the central
 // part, so B^T = R B R and
 //     L^T lam = R M L R M^{-1} lam
-// (M the global mass matrix, |J_k| M_ref per element).  Three launches: the
-// scaled inverse mass into the workspace, the plan's forward RHS kernel into
-// q_bar, the scaled mass in place on q_bar.  Measured on MI355X: wave P3
-// 259 -> 209 us, wave P4 366 -> 277 us, Maxwell P4 711 -> 516 us.
+// (M the global mass matrix, |J_k| M_ref per element).  The scaled inverse
+// mass goes into the workspace; then, for an MFMA plan, one launch of the MFMA
+// kernel with R |J| M_ref fused into its output phase writes q_bar, and for an
+// LDS-staged plan the forward kernel writes q_bar and a mass pass scales it in
+// place (the MFMA variant at Maxwell P2 is slower than both).  Measured on
+// MI355X, against the dedicated adjoint kernel: wave P3 259 -> 156 us, wave
+// P4 366 -> 220 us, Maxwell P4 711 -> 439 us, Maxwell P2 (LDS) 272 -> 274 us.
 //
 // Euler, triangles and the FP32 adjoint: mu = M_ref^{-1} lam into the
 // workspace, then the dedicated adjoint kernel (on triangles the two mass
@@ -1222,6 +1287,18 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
       adjoint_reflection(p, m.sign);
       hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
       DG_HIP_CHECK(hipGetLastError());
+      if (p->variant == DG_KERNEL_MFMA) {
+        // the MFMA kernel with R |J| M_ref applied to its output tile
+        RhsArgs r = rhs_args(p, (const double* const*)workspace, q_bar, false);
+        r.post_tiles = p->post_tiles;
+        r.jac = p->jac;
+        for (int c = 0; c < p->nfields; ++c) r.post_sign[c] = m.sign[c];
+        hipLaunchKernelGGL(p->post_kernel, dim3(p->post_grid), dim3(kBlock), 0, (hipStream_t)stream,
+                           r);
+        DG_HIP_CHECK(hipGetLastError());
+        return DG_OK;
+      }
+      // the plan's LDS-staged kernel, then R |J| M_ref in place
       if ((rc = dg_rhs(p, DG_PREC_FP64, (const double* const*)workspace, q_bar, stream)) != DG_OK)
         return rc;
       for (int c = 0; c < p->nfields; ++c) {

@@ -5,8 +5,9 @@
 // recomputed per node).  This file holds the per-element mass operator
 // (mass_apply_kernel) and the adjoint kernel (rhs_vjp_kernel).  The FP64
 // adjoint of wave and Maxwell on tetrahedra does not use the adjoint kernel:
-// the host runs it as R M L R M^{-1} lam on the forward kernel (dgfem.hip,
-// rhs_vjp_impl).
+// the host runs it as R M L R M^{-1} lam on the forward kernel, with the
+// M multiplication fused into the MFMA kernel (dgfem.hip, rhs_vjp_impl;
+// rhs_mfma.h, POST).
 //
 // The adjoint kernel path has two passes.  The first forms mu = M^{-1} lam per element.  Everything the
 // second pass needs is a function of mu:

@@ -178,7 +178,11 @@ __device__ constexpr int levi(int o, int x, int a) {
 // in flight without hoisting every load of the tile into registers.
 #define KUNROLL 3
 
-template <int EQ, int ORDER>
+// POST: the kernel multiplies each element's RHS by post_sign_c |J| M_ref
+// before storing it, one more contraction per field with the output tile as
+// the B operand -- the last step of the linear operators' adjoint
+// R M L R M^{-1} lam, fused so that it costs no pass over HBM.
+template <int EQ, int ORDER, bool POST = false>
 __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   using S = MfmaShape<ORDER>;
   using Flux = typename FluxOf<EQ>::type;
@@ -194,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
 
   __shared__ double s_op[S::OP_TILES * 64];
   __shared__ double s_lift[S::LIFT_TILES * 64];
+  __shared__ double s_post[POST ? MB * KS * 64 : 1];
   __shared__ int s_fmask[NFN];
   // per wave: scaled flux of C LIFT k-steps in MFMA B order [kk][field][FROW],
   // then one field of the output tile
@@ -211,6 +216,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   for (int k = tid; k < S::OP_TILES * 64; k += kBlock) s_op[k] = a.op_tiles[k];
   for (int k = tid; k < S::LIFT_TILES * 64; k += kBlock) s_lift[k] = a.lift_tiles[k];
   for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
+  if constexpr (POST)
+    for (int k = tid; k < MB * KS * 64; k += kBlock) s_post[k] = a.post_tiles[k];
   __syncthreads();
 
   const int lane = tid & 63;
@@ -487,6 +494,31 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           if (i < NP) buf[n * NP + i] = acc[c][b][v];
         }
       wave_lds_fence();
+      if constexpr (POST) {
+        // out[i][n] = sign_c |J_n| sum_j M_ref[i][j] out[j][n]: B operand
+        // node j = g * KS + ks of element n, read back from the buffer
+        f64x4 pacc[MB];
+#pragma unroll
+        for (int b = 0; b < MB; ++b) pacc[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int j = g * KS + ks;
+          const double bv = j < NP ? buf[n * NP + j] : 0.0;
+#pragma unroll
+          for (int b = 0; b < MB; ++b)
+            mma(pacc[b], s_post[(b * KS + ks) * 64 + lane], bv, SH && b == MB - 1);
+        }
+        wave_lds_fence();
+        const double sc = a.post_sign[c] * a.jac[el];
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * b + g + 4 * v;
+            if (i < NP) buf[n * NP + i] = sc * pacc[b][v];
+          }
+        wave_lds_fence();
+      }
       double* dst = a.rhs.p[c] + tile * 16 * NP;
       for (int idx = lane; idx < 16 * NP; idx += 64)
         if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);

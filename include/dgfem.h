@@ -165,11 +165,11 @@ int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
  * the exact transpose of dg_rhs for the linear wave and Maxwell operators
  * (q may then be NULL) and of dg_rhs linearised at q for Euler.  `workspace`
  * is nfields device buffers of (K, Np) float64 (overwritten).  Wave and
- * Maxwell on tetrahedra run three kernels on `stream` -- R M^{-1} rhs_bar
- * into the workspace, the dg_rhs kernel into q_bar, R M in place (the
- * identity L^T = R M L R M^{-1} of their upwind weak forms, see
- * dg_adjoint_space); Euler and triangle plans two: M_ref^{-1} rhs_bar, then
- * the adjoint kernel.  The reference has
+ * Maxwell on tetrahedra use the identity L^T = R M L R M^{-1} of their upwind
+ * weak forms (see dg_adjoint_space): R M^{-1} rhs_bar into the workspace,
+ * then the forward kernel with R M applied to its output (fused into the
+ * MFMA kernel; a separate pass after the LDS-staged one).  Euler and
+ * triangle plans: M_ref^{-1} rhs_bar, then the adjoint kernel.  The reference has
  * no adjoint: its jax:jit array context (run/__main__.py:213) is where
  * jax.vjp of the same RHS would come from.  Needs a plan created with
  * mass_inv and face_mass, else DG_ERR_UNSUPPORTED. */
