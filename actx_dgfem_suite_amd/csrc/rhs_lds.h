@@ -42,6 +42,20 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
   const int el = tid / NP;
   const int i = tid - el * NP;
   const double c0 = a.p0;
+  // Euler and Maxwell P2 keep each thread's rows of D_r^T in registers, so
+  // the contraction reads only the (broadcast) staged values from LDS.
+  // Measured on MI355X: Maxwell P2 -5%, Euler P2 -3%; wave P1-P2 +3-13% and
+  // P1/P3 even (registers cost blocks per CU).
+  constexpr bool ROWS = EQ != DG_EQ_WAVE && ORDER == 2;
+  constexpr int UNROLL_J = ROWS ? NP : 2;
+  double dr[3][ROWS ? NP : 1];
+  if constexpr (ROWS) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < NP; ++j) dr[r][j] = s_dt[r * NP * NP + j * NP + i];
+  }
 
   // Exterior values of the next batch are gathered into registers during
   // this batch's volume phase; R face-node slots per thread cover a batch.
@@ -177,9 +191,9 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
         double au = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll 4
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_dt[j * NP + i];
-          const double d1 = s_dt[NP * NP + j * NP + i];
-          const double d2 = s_dt[2 * NP * NP + j * NP + i];
+          const double d0 = ROWS ? dr[0][ROWS ? j : 0] : s_dt[j * NP + i];
+          const double d1 = ROWS ? dr[1][ROWS ? j : 0] : s_dt[NP * NP + j * NP + i];
+          const double d2 = ROWS ? dr[2][ROWS ? j : 0] : s_dt[2 * NP * NP + j * NP + i];
           const double uj = s_q[0][base + j];
           au += d0 * s_w[0][base + j] + d1 * s_w[1][base + j] + d2 * s_w[2][base + j];
           t0 += d0 * uj;
@@ -192,11 +206,11 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       } else if constexpr (EQ == DG_EQ_EULER) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) acc[c] = 0.0;
-#pragma unroll 2
+#pragma unroll UNROLL_J
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_dt[j * NP + i];
-          const double d1 = s_dt[NP * NP + j * NP + i];
-          const double d2 = s_dt[2 * NP * NP + j * NP + i];
+          const double d0 = ROWS ? dr[0][ROWS ? j : 0] : s_dt[j * NP + i];
+          const double d1 = ROWS ? dr[1][ROWS ? j : 0] : s_dt[NP * NP + j * NP + i];
+          const double d2 = ROWS ? dr[2][ROWS ? j : 0] : s_dt[2 * NP * NP + j * NP + i];
 #pragma unroll
           for (int c = 0; c < 5; ++c)
             acc[c] += d0 * s_w[3 * c][base + j] + d1 * s_w[3 * c + 1][base + j] +
@@ -206,11 +220,11 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
         double t[6][3];
 #pragma unroll
         for (int c = 0; c < 6; ++c) t[c][0] = t[c][1] = t[c][2] = 0.0;
-#pragma unroll 2
+#pragma unroll UNROLL_J
         for (int j = 0; j < NP; ++j) {
-          const double d0 = s_dt[j * NP + i];
-          const double d1 = s_dt[NP * NP + j * NP + i];
-          const double d2 = s_dt[2 * NP * NP + j * NP + i];
+          const double d0 = ROWS ? dr[0][ROWS ? j : 0] : s_dt[j * NP + i];
+          const double d1 = ROWS ? dr[1][ROWS ? j : 0] : s_dt[NP * NP + j * NP + i];
+          const double d2 = ROWS ? dr[2][ROWS ? j : 0] : s_dt[2 * NP * NP + j * NP + i];
 #pragma unroll
           for (int c = 0; c < 6; ++c) {
             const double fj = s_q[c][base + j];

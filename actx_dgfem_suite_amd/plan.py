@@ -210,8 +210,9 @@ class DGPlan:
         float64 fields use ``dg_rhs_vjp``; float32 fields the FP32 adjoint
         ``dg_rhs_vjp_fp32`` (FP64 arithmetic, FP32 storage).  ``q`` is read by
         Euler only (wave and Maxwell are linear) and may be ``None`` for them.
-        Launches on the current stream; the workspace is allocated on the
-        first call per dtype and reused."""
+        ``out`` may be ``rhs_bar`` itself, but not ``q``.  Launches on the
+        current stream; the workspace is allocated on the first call per dtype
+        and reused."""
         dtype = rhs_bar[0].dtype if len(rhs_bar) else torch.float64
         if dtype not in (torch.float64, torch.float32):
             raise TypeError(f"rhs_vjp: float64 or float32 fields, got {dtype}")

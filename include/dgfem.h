@@ -169,7 +169,9 @@ int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
  * weak forms (see dg_adjoint_space): R M^{-1} rhs_bar into the workspace,
  * then the forward kernel with R M applied to its output (fused into the
  * MFMA kernel; a separate pass after the LDS-staged one).  Euler and
- * triangle plans: M_ref^{-1} rhs_bar, then the adjoint kernel.  The reference has
+ * triangle plans: M_ref^{-1} rhs_bar, then the adjoint kernel.  q_bar may
+ * be rhs_bar, but not the workspace or q: the second kernel gathers
+ * neighbour values from those while it writes q_bar.  The reference has
  * no adjoint: its jax:jit array context (run/__main__.py:213) is where
  * jax.vjp of the same RHS would come from.  Needs a plan created with
  * mass_inv and face_mass, else DG_ERR_UNSUPPORTED. */
