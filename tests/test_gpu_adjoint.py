@@ -213,6 +213,23 @@ def test_linear_vjp_on_both_forward_kernels(equation, order, device):
     assert diff < 1e-13, diff
 
 
+@pytest.mark.parametrize("equation,order", [("wave", 3), ("maxwell", 2), ("euler", 2)])
+def test_vjp_in_place_on_the_cotangent(equation, order, device):
+    """``out`` may be the cotangent itself, on every VJP path (fused MFMA,
+    LDS-staged forward plus mass pass, Euler adjoint kernel): bit-identical
+    to the out-of-place result."""
+    print(f"[case] vjp in place {equation} P{order} n=3", flush=True)
+    d, od = discr_pair(order, 3)
+    plan = DGPlan(d, equation, PARAM[equation])
+    q = _dev(random_state(equation, od.x, seed=5), device) if equation == "euler" else None
+    y = _dev(np.random.default_rng(6).standard_normal((NF[equation], d.nelements, d.np)), device)
+    ref = [t.clone() for t in plan.rhs_vjp(q, y)]
+    y2 = [t.clone() for t in y]
+    got = plan.rhs_vjp(q, y2, out=y2)
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+
+
 def test_adjoint_space_rejects_euler(device):
     d, _ = discr_pair(1, 2)
     plan = DGPlan(d, "euler", PARAM["euler"])
