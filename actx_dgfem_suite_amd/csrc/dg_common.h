@@ -57,6 +57,12 @@ struct RhsArgs {
   const double* post_tiles;  // MFMA A tiles of M_ref: [mblock][kstep][64]
   const double* jac;         // (K) |J| per element
   double post_sign[kMaxFields];
+  // MFMA kernel with the LSERK4 stage fused into its output (STAGE), from
+  // the input state q: res = a * res + dt * rhs (res not read when a == 0);
+  // q_out = q + b * res.  q_out must not be q (other tiles gather from q).
+  OutPtrs stage_res;
+  OutPtrs stage_qout;
+  double stage_a, stage_b, stage_dt;
   int64_t nelem;
   double p0;
 };

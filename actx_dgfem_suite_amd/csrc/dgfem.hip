@@ -168,6 +168,26 @@ RhsKernel pick_mfma_post_order(int order) {
     default: return nullptr;
   }
 }
+// The MFMA kernel with the LSERK4 stage fused into its output.
+template <int EQ>
+RhsKernel pick_mfma_stage_order(int order) {
+  switch (order) {
+    case 1: return rhs_mfma_kernel<EQ, 1, false, true>;
+    case 2: return rhs_mfma_kernel<EQ, 2, false, true>;
+    case 3: return rhs_mfma_kernel<EQ, 3, false, true>;
+    case 4: return rhs_mfma_kernel<EQ, 4, false, true>;
+    default: return nullptr;
+  }
+}
+RhsKernel pick_mfma_stage(int eq, int order) {
+  switch (eq) {
+    case DG_EQ_WAVE: return pick_mfma_stage_order<DG_EQ_WAVE>(order);
+    case DG_EQ_EULER: return pick_mfma_stage_order<DG_EQ_EULER>(order);
+    case DG_EQ_MAXWELL: return pick_mfma_stage_order<DG_EQ_MAXWELL>(order);
+    default: return nullptr;
+  }
+}
+
 RhsKernel pick_mfma_post(int eq, int order) {
   switch (eq) {
     case DG_EQ_WAVE: return pick_mfma_post_order<DG_EQ_WAVE>(order);
@@ -670,6 +690,8 @@ struct dg_plan {
   double params[4] = {0, 0, 0, 0};
   int device = 0;
   int grid = 0;          // persistent grid of the RHS kernel
+  RhsKernel stage_kernel = nullptr;   // MFMA plans: the LSERK4 stage fused into the RHS
+  int stage_grid = 0;
   int lserk_grid = 0;
   KernelInfo kernel;
   // triangles: FP64 and FP32-storage forms of the 2D kernel
@@ -1068,6 +1090,17 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   grid = (grid + 7) / 8 * 8;
   p->grid = (int)grid;
   p->lserk_grid = prop.multiProcessorCount * 8;
+  if (!tri && p->variant == DG_KERNEL_MFMA) {
+    p->stage_kernel = pick_mfma_stage(d->equation, d->order);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->stage_kernel, kBlock,
+                                                      0) != hipSuccess ||
+        occ < 1)
+      occ = 1;
+    int64_t sg = (int64_t)prop.multiProcessorCount * occ;
+    if (sg > nbatch) sg = nbatch;
+    p->stage_grid = (int)((sg + 7) / 8 * 8);
+  }
   for (int k = 0; k < 2 && !tri; ++k) {
     const int prec = k == 0 ? DG_PREC_FP32 : DG_PREC_BF16X3;
     std::vector<uint32_t> ow, lw;
@@ -1189,6 +1222,36 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
   const RhsArgs a = rhs_args(p, in, out, p->variant == DG_KERNEL_LDS);
   hipLaunchKernelGGL(p->kernel.fn, dim3(p->grid), dim3(p->kernel.threads), 0,
                      (hipStream_t)stream, a);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
+int dg_rhs_lserk_stage(const dg_plan* p, double a, double b, double dt, const double* const* q_in,
+                       double* const* q_out, double* const* res, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->stage_kernel)
+    return fail(DG_ERR_UNSUPPORTED,
+                "dg_rhs_lserk_stage: tetrahedral plans on the MFMA kernel only (use dg_rhs and "
+                "dg_lserk_stage)");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)q_in, p->nfields, "q_in")) ||
+      (rc = check_ptrs((const void* const*)q_out, p->nfields, "q_out")) ||
+      (rc = check_ptrs((const void* const*)res, p->nfields, "res")))
+    return rc;
+  for (int c = 0; c < p->nfields; ++c)
+    for (int c2 = 0; c2 < p->nfields; ++c2)
+      if ((const void*)q_out[c] == (const void*)q_in[c2])
+        return fail(DG_ERR_INVALID_ARGUMENT, "dg_rhs_lserk_stage: q_out must not be q_in");
+  double* none[kMaxFields] = {};
+  RhsArgs r = rhs_args(p, q_in, none, false);
+  for (int c = 0; c < p->nfields; ++c) {
+    r.stage_res.p[c] = res[c];
+    r.stage_qout.p[c] = q_out[c];
+  }
+  r.stage_a = a;
+  r.stage_b = b;
+  r.stage_dt = dt;
+  hipLaunchKernelGGL(p->stage_kernel, dim3(p->stage_grid), dim3(kBlock), 0, (hipStream_t)stream, r);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
 }

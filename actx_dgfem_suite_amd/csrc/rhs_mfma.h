@@ -182,7 +182,10 @@ __device__ constexpr int levi(int o, int x, int a) {
 // before storing it, one more contraction per field with the output tile as
 // the B operand -- the last step of the linear operators' adjoint
 // R M L R M^{-1} lam, fused so that it costs no pass over HBM.
-template <int EQ, int ORDER, bool POST = false>
+// STAGE: instead of storing the RHS, the kernel applies the LSERK4 stage
+// res = a res + dt rhs, q_out = q + b res to it in its output phase, so the
+// RHS makes no round trip through HBM between the two.
+template <int EQ, int ORDER, bool POST = false, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   using S = MfmaShape<ORDER>;
   using Flux = typename FluxOf<EQ>::type;
@@ -484,8 +487,24 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
     // (non-temporal: the RHS is not read again by this kernel, and keeping it
     // out of L2 leaves room for the state the neighbours' gathers reuse)
 
+    // STAGE: each lane's slots of the output tile, idx = lane + 64 t
+    constexpr int NSLOT = (16 * NP + 63) / 64;
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
+      // STAGE: this field's res and q_in loads are issued before the tile
+      // goes through LDS, so they are in flight together
+      double st_res[STAGE ? NSLOT : 1], st_q[STAGE ? NSLOT : 1];
+      if constexpr (STAGE) {
+        const int64_t off = tile * 16 * NP;
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          const bool ok = idx < 16 * NP && tile * 16 + idx / NP < a.nelem;
+          // a == 0 (the first stage) does not read res, so it may start uninitialised
+          st_res[t] = ok && a.stage_a != 0.0 ? a.stage_res.p[c][off + idx] : 0.0;
+          st_q[t] = ok ? a.q.p[c][off + idx] : 0.0;
+        }
+      }
 #pragma unroll
       for (int b = 0; b < MB; ++b)
 #pragma unroll
@@ -519,9 +538,24 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           }
         wave_lds_fence();
       }
-      double* dst = a.rhs.p[c] + tile * 16 * NP;
-      for (int idx = lane; idx < 16 * NP; idx += 64)
-        if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+      if constexpr (STAGE) {
+        const int64_t off = tile * 16 * NP;
+        double* res = a.stage_res.p[c] + off;
+        double* qo = a.stage_qout.p[c] + off;
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem) {
+            const double r = (a.stage_a != 0.0 ? a.stage_a * st_res[t] : 0.0) + a.stage_dt * buf[idx];
+            res[idx] = r;
+            qo[idx] = st_q[t] + a.stage_b * r;
+          }
+        }
+      } else {
+        double* dst = a.rhs.p[c] + tile * 16 * NP;
+        for (int idx = lane; idx < 16 * NP; idx += 64)
+          if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+      }
       wave_lds_fence();
     }
 

@@ -203,6 +203,27 @@ class DGPlan:
             _lib.pointer_array([t.data_ptr() for t in rhs]),
             ctypes.c_void_p(stream)), "dg_lserk_stage")
 
+    @property
+    def fused_stage(self) -> bool:
+        """Whether ``rhs_lserk_stage`` is available: tetrahedral plans on the
+        MFMA kernel."""
+        return self.dim == 3 and self.kernel_variant == "mfma"
+
+    def rhs_lserk_stage(self, a: float, b: float, dt: float, q_in: Sequence[torch.Tensor],
+                        q_out: Sequence[torch.Tensor], res: Sequence[torch.Tensor]) -> None:
+        """One RHS and one LSERK4 stage in one kernel (``dg_rhs_lserk_stage``):
+        ``res = a*res + dt*L(q_in); q_out = q_in + b*res``.  ``q_out`` must be
+        other tensors than ``q_in``."""
+        for name, f in (("q_in", q_in), ("q_out", q_out), ("res", res)):
+            self._check_fields(f, name)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_rhs_lserk_stage(
+            self._handle, float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in q_in]),
+            _lib.pointer_array([t.data_ptr() for t in q_out]),
+            _lib.pointer_array([t.data_ptr() for t in res]),
+            ctypes.c_void_p(stream)), "dg_rhs_lserk_stage")
+
     def rhs_vjp(self, q: Sequence[torch.Tensor] | None, rhs_bar: Sequence[torch.Tensor],
                 out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:
         """Vector-Jacobian product ``q_bar = (dL/dq)^T rhs_bar`` at state ``q``.

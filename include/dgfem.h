@@ -152,6 +152,19 @@ int dg_lserk_stage(const dg_plan* plan, double a, double b, double dt,
                    double* const* q, double* const* res,
                    const double* const* rhs, void* stream);
 
+/* dg_rhs followed by dg_lserk_stage in one kernel, with the stage applied to
+ * the RHS in the kernel's output phase (no round trip of the RHS through
+ * HBM):
+ *   rhs = L(q_in);  res = a*res + dt*rhs;  q_out = q_in + b*res
+ * (a == 0 does not read res).  q_out must be other buffers than q_in, since
+ * the kernel gathers neighbour traces of q_in while it writes q_out; a
+ * stepper alternates two state buffers.  Tetrahedral plans on the MFMA
+ * kernel only; others return DG_ERR_UNSUPPORTED.  No counterpart in the
+ * reference (see dg_lserk_stage). */
+int dg_rhs_lserk_stage(const dg_plan* plan, double a, double b, double dt,
+                       const double* const* q_in, double* const* q_out,
+                       double* const* res, void* stream);
+
 /* LSERK4 stage for the mixed form: FP64 q and res, FP32 rhs in, and the FP32
  * shadow q32 = (float)q written for the next mixed RHS evaluation.  No
  * counterpart in the reference (see dg_lserk_stage). */

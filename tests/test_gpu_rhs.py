@@ -191,6 +191,75 @@ def test_lserk_stage_and_step_match_oracle(device):
     assert rel_l2(got, ref) < 1e-13
 
 
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_fused_lserk_stage_matches_rhs_then_stage(equation, order, device):
+    """dg_rhs_lserk_stage (the stage applied in the MFMA kernel's output)
+    against dg_rhs followed by dg_lserk_stage, over the five stages of a step
+    on a mesh with a partial last tile; and the fused LSERK4 stepper against
+    the oracle's steps: one ``step`` (copy back after five stages) and a
+    two-step ``run`` (ten stages, no copy)."""
+    print(f"[case] fused lserk {equation} P{order} n=3", flush=True)
+    d, od = discr_pair(order, 3)
+    param = {"wave": 1.0, "euler": 1.4, "maxwell": 0.5}[equation]
+    plan = DGPlan(d, equation, param, variant="mfma")
+    assert plan.fused_stage
+    q0 = random_state(equation, od.x, seed=order)
+    dt = 1e-3
+    qa = [torch.from_numpy(f.copy()).to(device) for f in q0]
+    qb = [t.clone() for t in qa]
+    qc = [torch.empty_like(t) for t in qa]
+    ra = [torch.full_like(t, float("nan")) for t in qa]   # a == 0 must not read res
+    rb = [t.clone() for t in ra]
+    rhs = plan.empty_fields()
+    for a, b in zip(dg_oracle.LSRK4_A, dg_oracle.LSRK4_B):
+        plan.rhs(qa, rhs)
+        plan.lserk_stage(a, b, dt, qa, ra, rhs)
+        plan.rhs_lserk_stage(a, b, dt, qb, qc, rb)
+        qb, qc = qc, qb
+    torch.cuda.synchronize()
+    got, ref = np.stack([t.cpu().numpy() for t in qb]), np.stack([t.cpu().numpy() for t in qa])
+    assert rel_l2(got, ref) < 1e-14
+    from actx_dgfem_suite_amd.timestepping import LSERK4
+    st = LSERK4(plan, dt, fused=True)
+    assert st.fused
+    qs = [torch.from_numpy(f.copy()).to(device) for f in q0]
+    st.step(qs)
+    torch.cuda.synchronize()
+    step = np.stack([t.cpu().numpy() for t in qs])
+    o1 = dg_oracle.lsrk4_step(lambda t, s: oracle_rhs(equation, od, s, param), q0, dt)
+    assert rel_l2(step, o1) < 1e-13
+    qs = [torch.from_numpy(f.copy()).to(device) for f in q0]
+    st.run(qs, 2)
+    torch.cuda.synchronize()
+    two = np.stack([t.cpu().numpy() for t in qs])
+    o2 = dg_oracle.lsrk4_step(lambda t, s: oracle_rhs(equation, od, s, param), o1, dt)
+    assert rel_l2(two, o2) < 1e-13
+
+
+def test_fused_lserk_stage_rejects_misuse(device):
+    d, _ = discr_pair(2, 2)
+    plan = DGPlan(d, "wave", 1.0, variant="mfma")
+    q = plan.empty_fields()
+    with pytest.raises(ValueError, match="q_out must not be q_in"):
+        plan.rhs_lserk_stage(0.0, 0.5, 1e-3, q, q, plan.empty_fields())
+    lds = DGPlan(d, "wave", 1.0, variant="lds")
+    assert not lds.fused_stage
+    with pytest.raises(NotImplementedError, match="MFMA"):
+        lds.rhs_lserk_stage(0.0, 0.5, 1e-3, q, plan.empty_fields(), plan.empty_fields())
+    from actx_dgfem_suite_amd.timestepping import LSERK4
+    with pytest.raises(ValueError, match="fused"):
+        LSERK4(lds, 1e-3, fused=True)
+    with pytest.raises(ValueError, match="fused"):
+        LSERK4(plan, 1e-3, precision="fp32", fused=True)
+    assert LSERK4(plan, 1e-3).fused and not LSERK4(lds, 1e-3).fused
+    assert not LSERK4(plan, 1e-3, fused=False).fused
+    # Maxwell P4 defaults to unfused stages (measured faster there)
+    d4, _ = discr_pair(4, 2)
+    mx4 = DGPlan(d4, "maxwell", 0.5, variant="mfma")
+    assert mx4.fused_stage and not LSERK4(mx4, 1e-3).fused
+
+
 def test_plan_rejects_bad_fields(device):
     d, _ = discr_pair(1, 2)
     plan = DGPlan(d, "wave", 1.0)
