@@ -805,6 +805,30 @@ int check_ptrs(const void* const* ptrs, int n, const char* what) {
   return DG_OK;
 }
 
+// Whether two fields of `bytes` bytes at a and b share memory.
+bool overlaps(const void* a, const void* b, size_t bytes) {
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + bytes && y < x + bytes;
+}
+
+// A kernel that gathers neighbour traces of its input fields while it writes
+// its output fields needs the outputs to share no memory with the inputs
+// (another wave may still read them) or with each other.
+int check_disjoint(const void* const* out, const char* out_name, const void* const* in,
+                   const char* in_name, int n, size_t bytes) {
+  for (int c = 0; c < n; ++c)
+    for (int c2 = 0; c2 < n; ++c2) {
+      if (overlaps(out[c], in[c2], bytes))
+        return fail(DG_ERR_INVALID_ARGUMENT, std::string(out_name) + " must not overlap " + in_name +
+                                                 " (" + out_name + "[" + std::to_string(c) + "], " +
+                                                 in_name + "[" + std::to_string(c2) + "])");
+      if (c2 > c && overlaps(out[c], out[c2], bytes))
+        return fail(DG_ERR_INVALID_ARGUMENT, std::string(out_name) + "[" + std::to_string(c) +
+                                                 "] and [" + std::to_string(c2) + "] overlap");
+    }
+  return DG_OK;
+}
+
 // Arguments of the tetrahedral FP64 RHS kernels (batch schedule for the
 // LDS-staged kernel, tile schedule for the MFMA kernels).
 RhsArgs rhs_args(const dg_plan* p, const double* const* in, double* const* out, bool batches) {
@@ -843,7 +867,9 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
                                         " not supported (FP32 or BF16X3)");
   int rc;
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
-      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
+      (rc = check_disjoint((const void* const*)out, "fields_out", (const void* const*)in,
+                           "fields_in", p->nfields, (size_t)(p->nelem * p->np) * sizeof(float))))
     return rc;
   if (p->dim == 2) {
     // triangles: FP32 state, every operation in FP64 (both precisions)
@@ -1206,7 +1232,9 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
     return fail(DG_ERR_UNSUPPORTED, "precision " + std::to_string(precision) + " not supported");
   int rc;
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
-      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
+      (rc = check_disjoint((const void* const*)out, "fields_out", (const void* const*)in,
+                           "fields_in", p->nfields, (size_t)(p->nelem * p->np) * sizeof(double))))
     return rc;
   if (p->dim == 2) {
     Rhs2DArgs<double> a{};
@@ -1238,10 +1266,14 @@ int dg_rhs_lserk_stage(const dg_plan* p, double a, double b, double dt, const do
       (rc = check_ptrs((const void* const*)q_out, p->nfields, "q_out")) ||
       (rc = check_ptrs((const void* const*)res, p->nfields, "res")))
     return rc;
-  for (int c = 0; c < p->nfields; ++c)
-    for (int c2 = 0; c2 < p->nfields; ++c2)
-      if ((const void*)q_out[c] == (const void*)q_in[c2])
-        return fail(DG_ERR_INVALID_ARGUMENT, "dg_rhs_lserk_stage: q_out must not be q_in");
+  const size_t bytes = (size_t)(p->nelem * p->np) * sizeof(double);
+  if ((rc = check_disjoint((const void* const*)q_out, "q_out", (const void* const*)q_in, "q_in",
+                           p->nfields, bytes)) ||
+      (rc = check_disjoint((const void* const*)res, "res", (const void* const*)q_in, "q_in",
+                           p->nfields, bytes)) ||
+      (rc = check_disjoint((const void* const*)res, "res", (const void* const*)q_out, "q_out",
+                           p->nfields, bytes)))
+    return rc;
   double* none[kMaxFields] = {};
   RhsArgs r = rhs_args(p, q_in, none, false);
   for (int c = 0; c < p->nfields; ++c) {

@@ -124,7 +124,11 @@ int dg_plan_variant(const dg_plan* plan, int32_t* variant);
 /* One right-hand-side evaluation, rhs = L(q): the call `compiled_rhs(*args)`
  * (run/__main__.py:76,109,126) of an array context, i.e. wave_operator
  * (equations/wave.py:121-200), EulerOperator.operator (equations/euler.py:
- * 176-179) or MaxwellOperator.operator (equations/maxwell.py:264-265). */
+ * 176-179) or MaxwellOperator.operator (equations/maxwell.py:264-265).
+ * The kernels gather neighbour traces of fields_in while they write
+ * fields_out, so an output field that shares memory with an input field or
+ * with another output field returns DG_ERR_INVALID_ARGUMENT (also for
+ * dg_rhs_mixed). */
 int dg_rhs(const dg_plan* plan, int32_t precision,
            const double* const* fields_in, double* const* fields_out,
            void* stream);
@@ -156,9 +160,10 @@ int dg_lserk_stage(const dg_plan* plan, double a, double b, double dt,
  * the RHS in the kernel's output phase (no round trip of the RHS through
  * HBM):
  *   rhs = L(q_in);  res = a*res + dt*rhs;  q_out = q_in + b*res
- * (a == 0 does not read res).  q_out must be other buffers than q_in, since
- * the kernel gathers neighbour traces of q_in while it writes q_out; a
- * stepper alternates two state buffers.  Tetrahedral plans on the MFMA
+ * (a == 0 does not read res).  q_out and res must share no memory with
+ * q_in, each other or themselves across fields (DG_ERR_INVALID_ARGUMENT
+ * otherwise), since the kernel gathers neighbour traces of q_in while it
+ * writes; a stepper alternates two state buffers.  Tetrahedral plans on the MFMA
  * kernel only; others return DG_ERR_UNSUPPORTED.  No counterpart in the
  * reference (see dg_lserk_stage). */
 int dg_rhs_lserk_stage(const dg_plan* plan, double a, double b, double dt,

@@ -409,7 +409,7 @@ def test_maxwell_adjoint_workload_gradient(device):
     qt = autodiff.trajectory(st, q, 100, checkpoint_every=10)
     j0 = objective(qt)
     grads = torch.autograd.grad(j0, q)
-    print(f"  J={float(j0):.10e}", flush=True)
+    print(f"  J={float(j0.detach()):.10e}", flush=True)
     rng = np.random.default_rng(2)
     worst = 0.0
     for k in range(16):

@@ -241,8 +241,20 @@ def test_fused_lserk_stage_rejects_misuse(device):
     d, _ = discr_pair(2, 2)
     plan = DGPlan(d, "wave", 1.0, variant="mfma")
     q = plan.empty_fields()
-    with pytest.raises(ValueError, match="q_out must not be q_in"):
+    with pytest.raises(ValueError, match="q_out must not overlap q_in"):
         plan.rhs_lserk_stage(0.0, 0.5, 1e-3, q, q, plan.empty_fields())
+    q2 = plan.empty_fields()
+    with pytest.raises(ValueError, match="res must not overlap q_in"):
+        plan.rhs_lserk_stage(0.0, 0.5, 1e-3, q, q2, q)
+    with pytest.raises(ValueError, match="res must not overlap q_out"):
+        plan.rhs_lserk_stage(0.0, 0.5, 1e-3, q, q2, q2)
+    # a partial overlap: q_out[0] starts half-way into q_in[1]
+    n = q[0].numel()
+    big = torch.empty(2 * n, dtype=torch.float64, device=device)
+    qa = [q[0], big[:n].view(plan.shape), q[2], q[3]]
+    qb = [big[n // 2:n // 2 + n].view(plan.shape), *plan.empty_fields()[1:]]
+    with pytest.raises(ValueError, match=r"q_out must not overlap q_in \(q_out\[0\], q_in\[1\]\)"):
+        plan.rhs_lserk_stage(0.0, 0.5, 1e-3, qa, qb, plan.empty_fields())
     lds = DGPlan(d, "wave", 1.0, variant="lds")
     assert not lds.fused_stage
     with pytest.raises(NotImplementedError, match="MFMA"):
@@ -272,6 +284,14 @@ def test_plan_rejects_bad_fields(device):
         plan.rhs([t[:, :2] for t in good])
     with pytest.raises(NotImplementedError):
         plan.rhs(good, precision="fp16")
+    # the kernels gather neighbour traces of the input while they write
+    with pytest.raises(ValueError, match="fields_out must not overlap fields_in"):
+        plan.rhs(good, good)
+    with pytest.raises(ValueError, match="overlap"):
+        plan.rhs(good, [good[0].clone()] * 4)
+    g32 = [t.float() for t in good]
+    with pytest.raises(ValueError, match="fields_out must not overlap fields_in"):
+        plan.rhs(g32, g32, precision="fp32")
 
 # }}}
 
