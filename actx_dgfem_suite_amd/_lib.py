@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "dg_rhs_mixed",
     "dg_lserk_stage",
     "dg_rhs_lserk_stage",
+    "dg_rhs_lserk_stage_mixed",
     "dg_lserk_stage_mixed",
     "dg_rhs_vjp",
     "dg_lserk_stage_adjoint",
@@ -108,6 +109,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_rhs_lserk_stage.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, _PP, _PP, _PP, ctypes.c_void_p]
     lib.dg_rhs_lserk_stage.restype = ctypes.c_int
+    lib.dg_rhs_lserk_stage_mixed.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, _PP, _PP, _PP, _PP,
+                                             ctypes.c_void_p]
+    lib.dg_rhs_lserk_stage_mixed.restype = ctypes.c_int
     lib.dg_adjoint_space.argtypes = [ctypes.c_void_p, ctypes.c_int32, _PP, _PP, ctypes.c_void_p]
     lib.dg_adjoint_space.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []

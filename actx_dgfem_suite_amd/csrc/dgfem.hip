@@ -222,30 +222,35 @@ TriKernel<T> pick_2d(int eq, int order) {
 
 using MixedKernel = void (*)(MixedArgs);
 
-template <int EQ, int PREC>
+template <int EQ, int PREC, bool STAGE>
 MixedKernel pick_mixed_order(int order) {
   switch (order) {
-    case 1: return rhs_mixed_kernel<EQ, PREC, 1>;
-    case 2: return rhs_mixed_kernel<EQ, PREC, 2>;
-    case 3: return rhs_mixed_kernel<EQ, PREC, 3>;
-    case 4: return rhs_mixed_kernel<EQ, PREC, 4>;
+    case 1: return rhs_mixed_kernel<EQ, PREC, 1, STAGE>;
+    case 2: return rhs_mixed_kernel<EQ, PREC, 2, STAGE>;
+    case 3: return rhs_mixed_kernel<EQ, PREC, 3, STAGE>;
+    case 4: return rhs_mixed_kernel<EQ, PREC, 4, STAGE>;
     default: return nullptr;
   }
 }
 
-template <int PREC>
+template <int PREC, bool STAGE>
 MixedKernel pick_mixed_prec(int eq, int order) {
   switch (eq) {
-    case DG_EQ_WAVE: return pick_mixed_order<DG_EQ_WAVE, PREC>(order);
-    case DG_EQ_EULER: return pick_mixed_order<DG_EQ_EULER, PREC>(order);
-    case DG_EQ_MAXWELL: return pick_mixed_order<DG_EQ_MAXWELL, PREC>(order);
+    case DG_EQ_WAVE: return pick_mixed_order<DG_EQ_WAVE, PREC, STAGE>(order);
+    case DG_EQ_EULER: return pick_mixed_order<DG_EQ_EULER, PREC, STAGE>(order);
+    case DG_EQ_MAXWELL: return pick_mixed_order<DG_EQ_MAXWELL, PREC, STAGE>(order);
     default: return nullptr;
   }
 }
 
-MixedKernel pick_mixed(int prec, int eq, int order) {
-  if (prec == DG_PREC_FP32) return pick_mixed_prec<DG_PREC_FP32>(eq, order);
-  if (prec == DG_PREC_BF16X3) return pick_mixed_prec<DG_PREC_BF16X3>(eq, order);
+// The mixed RHS kernel, or (stage) its variant with the LSERK4 stage fused.
+MixedKernel pick_mixed(int prec, int eq, int order, bool stage = false) {
+  if (prec == DG_PREC_FP32)
+    return stage ? pick_mixed_prec<DG_PREC_FP32, true>(eq, order)
+                 : pick_mixed_prec<DG_PREC_FP32, false>(eq, order);
+  if (prec == DG_PREC_BF16X3)
+    return stage ? pick_mixed_prec<DG_PREC_BF16X3, true>(eq, order)
+                 : pick_mixed_prec<DG_PREC_BF16X3, false>(eq, order);
   return nullptr;
 }
 
@@ -714,6 +719,9 @@ struct dg_plan {
   uint32_t* mixed_lift[2] = {nullptr, nullptr};
   MixedKernel mixed_kernel[2] = {nullptr, nullptr};
   int mixed_grid[2] = {0, 0};
+  // tetrahedra: the mixed kernels with the LSERK4 stage fused
+  MixedKernel mixed_stage_kernel[2] = {nullptr, nullptr};
+  int mixed_stage_grid[2] = {0, 0};
   // locality schedules: 16-element tiles (MFMA and mixed kernels) and
   // batches of kBlock / Np elements (LDS-staged and adjoint kernels)
   int32_t* sched_tile = nullptr;
@@ -805,24 +813,24 @@ int check_ptrs(const void* const* ptrs, int n, const char* what) {
   return DG_OK;
 }
 
-// Whether two fields of `bytes` bytes at a and b share memory.
-bool overlaps(const void* a, const void* b, size_t bytes) {
+// Whether fields of na bytes at a and nb bytes at b share memory.
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
   const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
-  return x < y + bytes && y < x + bytes;
+  return x < y + nb && y < x + na;
 }
 
 // A kernel that gathers neighbour traces of its input fields while it writes
 // its output fields needs the outputs to share no memory with the inputs
 // (another wave may still read them) or with each other.
-int check_disjoint(const void* const* out, const char* out_name, const void* const* in,
-                   const char* in_name, int n, size_t bytes) {
+int check_disjoint(const void* const* out, const char* out_name, size_t out_bytes,
+                   const void* const* in, const char* in_name, size_t in_bytes, int n) {
   for (int c = 0; c < n; ++c)
     for (int c2 = 0; c2 < n; ++c2) {
-      if (overlaps(out[c], in[c2], bytes))
+      if (overlaps(out[c], out_bytes, in[c2], in_bytes))
         return fail(DG_ERR_INVALID_ARGUMENT, std::string(out_name) + " must not overlap " + in_name +
                                                  " (" + out_name + "[" + std::to_string(c) + "], " +
                                                  in_name + "[" + std::to_string(c2) + "])");
-      if (c2 > c && overlaps(out[c], out[c2], bytes))
+      if (c2 > c && overlaps(out[c], out_bytes, out[c2], out_bytes))
         return fail(DG_ERR_INVALID_ARGUMENT, std::string(out_name) + "[" + std::to_string(c) +
                                                  "] and [" + std::to_string(c2) + "] overlap");
     }
@@ -853,37 +861,8 @@ RhsArgs rhs_args(const dg_plan* p, const double* const* in, double* const* out, 
   return a;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* dg_last_error_string(void) { return g_last_error.c_str(); }
-
-int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, float* const* out,
-                 void* stream) {
-  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
-  if (precision != DG_PREC_FP32 && precision != DG_PREC_BF16X3)
-    return fail(DG_ERR_UNSUPPORTED, "mixed precision " + std::to_string(precision) +
-                                        " not supported (FP32 or BF16X3)");
-  int rc;
-  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
-      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
-      (rc = check_disjoint((const void* const*)out, "fields_out", (const void* const*)in,
-                           "fields_in", p->nfields, (size_t)(p->nelem * p->np) * sizeof(float))))
-    return rc;
-  if (p->dim == 2) {
-    // triangles: FP32 state, every operation in FP64 (both precisions)
-    Rhs2DArgs<float> a{};
-    for (int c = 0; c < p->nfields; ++c) {
-      a.q[c] = in[c];
-      a.rhs[c] = out[c];
-    }
-    fill_2d_args(p, a);
-    hipLaunchKernelGGL(p->tri_kernel32, dim3(p->grid), dim3(kBlock), 0, (hipStream_t)stream, a);
-    DG_HIP_CHECK(hipGetLastError());
-    return DG_OK;
-  }
-  const int k = precision == DG_PREC_FP32 ? 0 : 1;
+// Arguments of the tetrahedral mixed kernels (precision slot k: 0 FP32, 1 BF16x3).
+MixedArgs mixed_args(const dg_plan* p, int k, const float* const* in, float* const* out) {
   MixedArgs a{};
   for (int c = 0; c < p->nfields; ++c) {
     a.q[c] = in[c];
@@ -900,8 +879,93 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
   a.lift_tiles = p->mixed_lift[k];
   a.nelem = p->nelem;
   a.p0 = p->params[0];
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dg_last_error_string(void) { return g_last_error.c_str(); }
+
+int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, float* const* out,
+                 void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (precision != DG_PREC_FP32 && precision != DG_PREC_BF16X3)
+    return fail(DG_ERR_UNSUPPORTED, "mixed precision " + std::to_string(precision) +
+                                        " not supported (FP32 or BF16X3)");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
+      (rc = check_disjoint((const void* const*)out, "fields_out",
+                           (size_t)(p->nelem * p->np) * sizeof(float), (const void* const*)in,
+                           "fields_in", (size_t)(p->nelem * p->np) * sizeof(float), p->nfields)))
+    return rc;
+  if (p->dim == 2) {
+    // triangles: FP32 state, every operation in FP64 (both precisions)
+    Rhs2DArgs<float> a{};
+    for (int c = 0; c < p->nfields; ++c) {
+      a.q[c] = in[c];
+      a.rhs[c] = out[c];
+    }
+    fill_2d_args(p, a);
+    hipLaunchKernelGGL(p->tri_kernel32, dim3(p->grid), dim3(kBlock), 0, (hipStream_t)stream, a);
+    DG_HIP_CHECK(hipGetLastError());
+    return DG_OK;
+  }
+  const int k = precision == DG_PREC_FP32 ? 0 : 1;
+  const MixedArgs a = mixed_args(p, k, in, out);
   hipLaunchKernelGGL(p->mixed_kernel[k], dim3(p->mixed_grid[k]), dim3(kBlock), 0,
                      (hipStream_t)stream, a);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
+int dg_rhs_lserk_stage_mixed(const dg_plan* p, int32_t precision, double a, double b, double dt,
+                             const float* const* q32_in, float* const* q32_out, double* const* q,
+                             double* const* res, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (precision != DG_PREC_FP32 && precision != DG_PREC_BF16X3)
+    return fail(DG_ERR_UNSUPPORTED, "mixed precision " + std::to_string(precision) +
+                                        " not supported (FP32 or BF16X3)");
+  if (p->dim == 2)
+    return fail(DG_ERR_UNSUPPORTED,
+                "dg_rhs_lserk_stage_mixed: tetrahedral plans only (use dg_rhs_mixed and "
+                "dg_lserk_stage_mixed)");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)q32_in, p->nfields, "q32_in")) ||
+      (rc = check_ptrs((const void* const*)q32_out, p->nfields, "q32_out")) ||
+      (rc = check_ptrs((const void* const*)q, p->nfields, "q")) ||
+      (rc = check_ptrs((const void* const*)res, p->nfields, "res")))
+    return rc;
+  const size_t n = (size_t)(p->nelem * p->np);
+  const size_t b4 = n * sizeof(float), b8 = n * sizeof(double);
+  if ((rc = check_disjoint((const void* const*)q32_out, "q32_out", b4,
+                           (const void* const*)q32_in, "q32_in", b4, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)q, "q", b8, (const void* const*)q32_in, "q32_in", b4,
+                           p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, "res", b8, (const void* const*)q32_in,
+                           "q32_in", b4, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, "res", b8, (const void* const*)q, "q", b8,
+                           p->nfields)) ||
+      (rc = check_disjoint((const void* const*)q32_out, "q32_out", b4, (const void* const*)q, "q",
+                           b8, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)q32_out, "q32_out", b4, (const void* const*)res,
+                           "res", b8, p->nfields)))
+    return rc;
+  const int k = precision == DG_PREC_FP32 ? 0 : 1;
+  float* none[kMaxFields] = {};
+  MixedArgs m = mixed_args(p, k, q32_in, none);
+  for (int c = 0; c < p->nfields; ++c) {
+    m.st_q[c] = q[c];
+    m.st_res[c] = res[c];
+    m.st_q32[c] = q32_out[c];
+  }
+  m.st_a = a;
+  m.st_b = b;
+  m.st_dt = dt;
+  hipLaunchKernelGGL(p->mixed_stage_kernel[k], dim3(p->mixed_stage_grid[k]), dim3(kBlock), 0,
+                     (hipStream_t)stream, m);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
 }
@@ -1137,15 +1201,19 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       return rc;
     }
     p->mixed_kernel[k] = pick_mixed(prec, d->equation, d->order);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->mixed_kernel[k], kBlock,
-                                                      0) != hipSuccess ||
-        occ < 1)
-      occ = 1;
-    int64_t mg = (int64_t)prop.multiProcessorCount * occ;
-    const int64_t mbatch = (((d->nelements + 15) / 16) + 3) / 4;
-    if (mg > mbatch) mg = mbatch;
-    p->mixed_grid[k] = (int)((mg + 7) / 8 * 8);
+    p->mixed_stage_kernel[k] = pick_mixed(prec, d->equation, d->order, true);
+    for (int st = 0; st < 2; ++st) {
+      const MixedKernel fn = st ? p->mixed_stage_kernel[k] : p->mixed_kernel[k];
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)fn, kBlock, 0) !=
+              hipSuccess ||
+          occ < 1)
+        occ = 1;
+      int64_t mg = (int64_t)prop.multiProcessorCount * occ;
+      const int64_t mbatch = (((d->nelements + 15) / 16) + 3) / 4;
+      if (mg > mbatch) mg = mbatch;
+      (st ? p->mixed_stage_grid[k] : p->mixed_grid[k]) = (int)((mg + 7) / 8 * 8);
+    }
   }
   if (want_vjp) {
     if ((rc = upload(&p->amat_t, amat_t.data(), amat_t.size())) ||
@@ -1233,8 +1301,9 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
   int rc;
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
       (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
-      (rc = check_disjoint((const void* const*)out, "fields_out", (const void* const*)in,
-                           "fields_in", p->nfields, (size_t)(p->nelem * p->np) * sizeof(double))))
+      (rc = check_disjoint((const void* const*)out, "fields_out",
+                           (size_t)(p->nelem * p->np) * sizeof(double), (const void* const*)in,
+                           "fields_in", (size_t)(p->nelem * p->np) * sizeof(double), p->nfields)))
     return rc;
   if (p->dim == 2) {
     Rhs2DArgs<double> a{};
@@ -1267,12 +1336,12 @@ int dg_rhs_lserk_stage(const dg_plan* p, double a, double b, double dt, const do
       (rc = check_ptrs((const void* const*)res, p->nfields, "res")))
     return rc;
   const size_t bytes = (size_t)(p->nelem * p->np) * sizeof(double);
-  if ((rc = check_disjoint((const void* const*)q_out, "q_out", (const void* const*)q_in, "q_in",
-                           p->nfields, bytes)) ||
-      (rc = check_disjoint((const void* const*)res, "res", (const void* const*)q_in, "q_in",
-                           p->nfields, bytes)) ||
-      (rc = check_disjoint((const void* const*)res, "res", (const void* const*)q_out, "q_out",
-                           p->nfields, bytes)))
+  if ((rc = check_disjoint((const void* const*)q_out, "q_out", bytes, (const void* const*)q_in,
+                           "q_in", bytes, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, "res", bytes, (const void* const*)q_in, "q_in",
+                           bytes, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, "res", bytes, (const void* const*)q_out,
+                           "q_out", bytes, p->nfields)))
     return rc;
   double* none[kMaxFields] = {};
   RhsArgs r = rhs_args(p, q_in, none, false);

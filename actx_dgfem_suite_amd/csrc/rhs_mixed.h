@@ -48,6 +48,14 @@ struct MixedArgs {
   const void* lift_tiles;   // A tiles of LIFT
   int64_t nelem;
   double p0;
+  // STAGE only: the LSERK4 stage applied to the RHS tile -- the FP64 state
+  // and residual of the tile's own elements are updated in place, and the
+  // new FP32 shadow goes to st_q32 (not q, whose neighbour traces other
+  // waves may still gather)
+  double* st_q[kMaxFields];
+  double* st_res[kMaxFields];
+  float* st_q32[kMaxFields];
+  double st_a, st_b, st_dt;
 };
 
 // round-to-nearest-even float -> bf16 bits (finite inputs)
@@ -166,7 +174,10 @@ constexpr bool mixed_tile_tables() {
   return ORDER <= 2;
 }
 
-template <int EQ, int PREC, int ORDER>
+// STAGE: instead of storing the FP32 RHS, the kernel applies the stage
+// res = a res + dt rhs, q = q + b res, q32 = (float)q to it in its output
+// phase (the FP32 RHS tile is the one the unfused path would store).
+template <int EQ, int PREC, int ORDER, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   using S = MixedShape<PREC, ORDER>;
   using M = MixedMma<PREC>;
@@ -488,8 +499,24 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 
     // {{{ output through the buffer, one field at a time
 
+    // STAGE: each lane's slots of the output tile, idx = lane + 64 t
+    constexpr int NSLOT = (16 * NP + 63) / 64;
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
+      // STAGE: this field's res and q loads are issued before the tile goes
+      // through LDS, so they are in flight together
+      double st_res[STAGE ? NSLOT : 1], st_q[STAGE ? NSLOT : 1];
+      if constexpr (STAGE) {
+        const int64_t off = tile * 16 * NP;
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          const bool ok = idx < 16 * NP && tile * 16 + idx / NP < a.nelem;
+          // a == 0 (the first stage) does not read res, so it may start uninitialised
+          st_res[t] = ok && a.st_a != 0.0 ? a.st_res[c][off + idx] : 0.0;
+          st_q[t] = ok ? a.st_q[c][off + idx] : 0.0;
+        }
+      }
 #pragma unroll
       for (int b = 0; b < MB; ++b)
 #pragma unroll
@@ -498,9 +525,24 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
           if (i < NP) buf[n * NP + i] = acc[c][b][v];
         }
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      float* dst = a.rhs[c] + tile * 16 * NP;
-      for (int idx = lane; idx < 16 * NP; idx += 64)
-        if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+      if constexpr (STAGE) {
+        const int64_t off = tile * 16 * NP;
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem) {
+            const double r = (a.st_a != 0.0 ? a.st_a * st_res[t] : 0.0) + a.st_dt * (double)buf[idx];
+            const double qn = st_q[t] + a.st_b * r;
+            a.st_res[c][off + idx] = r;
+            a.st_q[c][off + idx] = qn;
+            a.st_q32[c][off + idx] = (float)qn;
+          }
+        }
+      } else {
+        float* dst = a.rhs[c] + tile * 16 * NP;
+        for (int idx = lane; idx < 16 * NP; idx += 64)
+          if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+      }
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     }
 

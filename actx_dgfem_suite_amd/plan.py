@@ -212,8 +212,8 @@ class DGPlan:
     def rhs_lserk_stage(self, a: float, b: float, dt: float, q_in: Sequence[torch.Tensor],
                         q_out: Sequence[torch.Tensor], res: Sequence[torch.Tensor]) -> None:
         """One RHS and one LSERK4 stage in one kernel (``dg_rhs_lserk_stage``):
-        ``res = a*res + dt*L(q_in); q_out = q_in + b*res``.  ``q_out`` must be
-        other tensors than ``q_in``."""
+        ``res = a*res + dt*L(q_in); q_out = q_in + b*res``.  ``q_out`` and
+        ``res`` must share no memory with ``q_in`` or each other."""
         for name, f in (("q_in", q_in), ("q_out", q_out), ("res", res)):
             self._check_fields(f, name)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -223,6 +223,32 @@ class DGPlan:
             _lib.pointer_array([t.data_ptr() for t in q_out]),
             _lib.pointer_array([t.data_ptr() for t in res]),
             ctypes.c_void_p(stream)), "dg_rhs_lserk_stage")
+
+    @property
+    def fused_stage_mixed(self) -> bool:
+        """Whether ``rhs_lserk_stage_mixed`` is available: tetrahedral plans."""
+        return self.dim == 3
+
+    def rhs_lserk_stage_mixed(self, precision: str, a: float, b: float, dt: float,
+                              q32_in: Sequence[torch.Tensor], q32_out: Sequence[torch.Tensor],
+                              q: Sequence[torch.Tensor], res: Sequence[torch.Tensor]) -> None:
+        """One mixed RHS and one LSERK4 stage in one kernel
+        (``dg_rhs_lserk_stage_mixed``): ``res = a*res + dt*L(q32_in); q +=
+        b*res; q32_out = float(q)``, with ``q`` and ``res`` float64 (updated
+        in place) and ``q32_in``/``q32_out`` two different float32 shadows."""
+        if precision not in _lib.MIXED_PRECISIONS:
+            raise NotImplementedError(f"mixed precision '{precision}'")
+        for name, f, dt_ in (("q32_in", q32_in, torch.float32), ("q32_out", q32_out, torch.float32),
+                             ("q", q, torch.float64), ("res", res, torch.float64)):
+            self._check_fields(f, name, dt_)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_rhs_lserk_stage_mixed(
+            self._handle, _lib.PRECISION_IDS[precision], float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in q32_in]),
+            _lib.pointer_array([t.data_ptr() for t in q32_out]),
+            _lib.pointer_array([t.data_ptr() for t in q]),
+            _lib.pointer_array([t.data_ptr() for t in res]),
+            ctypes.c_void_p(stream)), "dg_rhs_lserk_stage_mixed")
 
     def rhs_vjp(self, q: Sequence[torch.Tensor] | None, rhs_bar: Sequence[torch.Tensor],
                 out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:

@@ -56,16 +56,33 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
     return [t.clone() for t in fields]
 
 
-def fused_stage_preferred(plan) -> bool:
-    """Default of ``LSERK4(fused=None)``: fused stages where the plan has
-    them, except Maxwell P4.
+# Mixed forms whose fused step measured at least 4% faster on MI355X (10
+# whole steps through ``run`` on meshes of 94k-750k tets, fused against the
+# mixed RHS and stage kernels).  FP32: wave P3 1.08x, P4 1.14x, Euler P1
+# 1.06x.  BF16x3: wave P2 1.08x, P3 1.05x, P4 1.12x, Euler P3 1.12x, Maxwell
+# P3 1.07x, P4 1.06x.  Elsewhere it was slower (0.69x-0.98x; Euler P2 BF16x3
+# 1.02x), mostly where the stage epilogue's registers cost a wave per SIMD.
+_FUSED_MIXED = {
+    "fp32": {("wave", 3), ("wave", 4), ("euler", 1)},
+    "bf16x3": {("wave", 2), ("wave", 3), ("wave", 4), ("euler", 3), ("maxwell", 3),
+               ("maxwell", 4)},
+}
 
-    Measured on MI355X over 20 whole steps (``run``), fused against RHS +
+
+def fused_stage_preferred(plan, precision: str = "fp64") -> bool:
+    """Default of ``LSERK4(fused=None)``.  Mixed forms: fused stages on the
+    tetrahedral cases in ``_FUSED_MIXED``.  FP64: fused stages where the plan
+    has them, except Maxwell P4.
+
+    FP64, measured on MI355X over 20 whole steps (``run``), fused against RHS +
     stage kernels, on meshes of 94k-750k tets: 1.01-1.14x faster for wave and
     Euler P1-P4 and Maxwell P1 and P3, 0.987x for Maxwell P4.  Unfused, the
     stage kernel re-reads the state and RHS the RHS kernel has just touched
     while they are still in the 256 MB Infinity Cache, which takes back most
     of what fusing saves in HBM traffic for Maxwell P4's six fields."""
+    if precision != "fp64":
+        return bool(plan.fused_stage_mixed) and \
+            (plan.equation, plan.discr.order) in _FUSED_MIXED[precision]
     return bool(plan.fused_stage) and not (plan.equation == "maxwell" and plan.discr.order == 4)
 
 
@@ -83,22 +100,25 @@ class LSERK4:
     def __init__(self, plan, dt: float, precision: str = "fp64", fused: bool | None = None):
         if precision not in ("fp64", "fp32", "bf16x3"):
             raise ValueError(f"precision '{precision}'")
-        if fused and (precision != "fp64" or not plan.fused_stage):
-            raise ValueError("fused stages need an FP64 stepper on a plan with plan.fused_stage")
+        available = plan.fused_stage if precision == "fp64" else plan.fused_stage_mixed
+        if fused and not available:
+            raise ValueError("fused stages need plan.fused_stage (FP64) or "
+                             "plan.fused_stage_mixed (mixed forms)")
         self.plan = plan
         self.dt = float(dt)
         self.precision = precision
         self._res = plan.empty_fields()
         if fused is None:
-            fused = precision == "fp64" and fused_stage_preferred(plan)
+            fused = fused_stage_preferred(plan, precision)
         # fused: the stage runs inside the RHS kernel, which reads one state
-        # buffer and writes the other
-        self._q2 = plan.empty_fields() if fused else None
+        # buffer (FP64) or FP32 shadow (mixed) and writes the other
+        self._q2 = plan.empty_fields() if fused and precision == "fp64" else None
+        self._q32b = plan.empty_fields(torch.float32) if fused and precision != "fp64" else None
         if precision == "fp64":
             self._rhs = None if self._q2 is not None else plan.empty_fields()
             self._q32 = None
         else:
-            self._rhs = plan.empty_fields(torch.float32)
+            self._rhs = None if self._q32b is not None else plan.empty_fields(torch.float32)
             self._q32 = plan.empty_fields(torch.float32)
         self._adj: dict[torch.dtype, tuple[list[torch.Tensor], list[torch.Tensor]]] = {}
 
@@ -124,6 +144,16 @@ class LSERK4:
             return
         for t, src in zip(self._q32, q):
             t.copy_(src)
+        if self._q32b is not None:
+            # the FP64 state and residual are updated in place; the kernel
+            # gathers one FP32 shadow and writes the other
+            s32, d32 = self._q32, self._q32b
+            for a, b in zip(LSERK4_A, LSERK4_B):
+                if stages is not None:
+                    stages.append(_clone(q))
+                plan.rhs_lserk_stage_mixed(self.precision, a, b, self.dt, s32, d32, q, self._res)
+                s32, d32 = d32, s32
+            return
         for a, b in zip(LSERK4_A, LSERK4_B):
             if stages is not None:
                 stages.append(_clone(q))
@@ -132,8 +162,9 @@ class LSERK4:
 
     @property
     def fused(self) -> bool:
-        """Whether the FP64 steps run fused stages (``dg_rhs_lserk_stage``)."""
-        return self._q2 is not None
+        """Whether the steps run fused stages (``dg_rhs_lserk_stage`` or
+        ``dg_rhs_lserk_stage_mixed``)."""
+        return self._q2 is not None or self._q32b is not None
 
     def _fused_steps(self, q: Sequence[torch.Tensor], nsteps: int,
                      stages: list[list[torch.Tensor]] | None = None) -> None:

@@ -178,6 +178,22 @@ int dg_lserk_stage_mixed(const dg_plan* plan, double a, double b, double dt,
                          const float* const* rhs, float* const* q32,
                          void* stream);
 
+/* dg_rhs_mixed followed by dg_lserk_stage_mixed in one kernel, the stage
+ * applied to the FP32 RHS tile in the kernel's output phase:
+ *   rhs = L(q32_in) (FP32 or BF16X3);  res = a*res + dt*rhs;
+ *   q = q + b*res;  q32_out = (float)q
+ * (a == 0 does not read res).  The FP64 q and res are updated in place; the
+ * kernel gathers neighbour traces of q32_in while it writes, so q32_out, q
+ * and res must share no memory with q32_in or with each other
+ * (DG_ERR_INVALID_ARGUMENT otherwise), and a stepper alternates two FP32
+ * shadow buffers.  Tetrahedral plans only; triangle plans return
+ * DG_ERR_UNSUPPORTED.  No counterpart in the reference (see
+ * dg_lserk_stage). */
+int dg_rhs_lserk_stage_mixed(const dg_plan* plan, int32_t precision, double a,
+                             double b, double dt, const float* const* q32_in,
+                             float* const* q32_out, double* const* q,
+                             double* const* res, void* stream);
+
 /* Vector-Jacobian product of the FP64 right-hand side at state q:
  *   q_bar = (dL/dq)^T rhs_bar
  * the exact transpose of dg_rhs for the linear wave and Maxwell operators

@@ -13,7 +13,14 @@ from actx_dgfem_suite_amd.discretization import make_box_discretization
 from actx_dgfem_suite_amd.plan import DGPlan
 from actx_dgfem_suite_amd.workloads import WORKLOADS, initial_state
 from oracle import dg_oracle
-from tests._helpers import discr_pair, field_errors, oracle_rhs, random_state, rel_l2
+from tests._helpers import (
+    discr_pair,
+    discr_pair_2d,
+    field_errors,
+    oracle_rhs,
+    random_state,
+    rel_l2,
+)
 
 pytestmark = pytest.mark.gpu
 
@@ -262,14 +269,88 @@ def test_fused_lserk_stage_rejects_misuse(device):
     from actx_dgfem_suite_amd.timestepping import LSERK4
     with pytest.raises(ValueError, match="fused"):
         LSERK4(lds, 1e-3, fused=True)
+    d2, _ = discr_pair_2d(2, 2)
+    tri = DGPlan(d2, "wave", 1.0)
     with pytest.raises(ValueError, match="fused"):
-        LSERK4(plan, 1e-3, precision="fp32", fused=True)
+        LSERK4(tri, 1e-3, precision="fp32", fused=True)
     assert LSERK4(plan, 1e-3).fused and not LSERK4(lds, 1e-3).fused
     assert not LSERK4(plan, 1e-3, fused=False).fused
     # Maxwell P4 defaults to unfused stages (measured faster there)
     d4, _ = discr_pair(4, 2)
     mx4 = DGPlan(d4, "maxwell", 0.5, variant="mfma")
     assert mx4.fused_stage and not LSERK4(mx4, 1e-3).fused
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("equation", ["wave", "euler", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_fused_mixed_stage_matches_unfused(precision, equation, order, device):
+    """dg_rhs_lserk_stage_mixed against dg_rhs_mixed followed by
+    dg_lserk_stage_mixed over the five stages of a step, on a mesh with a
+    partial last tile: both apply the stage to the same FP32 RHS, so they
+    differ by the rounding of the FP64 update only; the new FP32 shadow is
+    the FP64 state rounded.  And the fused mixed LSERK4 stepper against the
+    unfused one."""
+    print(f"[case] fused mixed lserk {precision} {equation} P{order} n=3", flush=True)
+    d, od = discr_pair(order, 3)
+    param = {"wave": 1.0, "euler": 1.4, "maxwell": 0.5}[equation]
+    plan = DGPlan(d, equation, param)
+    assert plan.fused_stage_mixed
+    q0 = random_state(equation, od.x, seed=order)
+    dt = 1e-3
+    qa = [torch.from_numpy(f.copy()).to(device) for f in q0]
+    qb = [t.clone() for t in qa]
+    ra = [torch.full_like(t, float("nan")) for t in qa]   # a == 0 must not read res
+    rb = [t.clone() for t in ra]
+    a32 = [t.float() for t in qa]
+    s32, d32 = [t.float() for t in qb], plan.empty_fields(torch.float32)
+    rhs32 = plan.empty_fields(torch.float32)
+    for a, b in zip(dg_oracle.LSRK4_A, dg_oracle.LSRK4_B):
+        plan.rhs_mixed(a32, rhs32, precision)
+        plan.lserk_stage_mixed(a, b, dt, qa, ra, rhs32, a32)
+        plan.rhs_lserk_stage_mixed(precision, a, b, dt, s32, d32, qb, rb)
+        s32, d32 = d32, s32
+    torch.cuda.synchronize()
+    st = lambda f: np.stack([t.cpu().numpy() for t in f])  # noqa: E731
+    assert rel_l2(st(qb), st(qa)) < 1e-14
+    assert rel_l2(st(rb), st(ra)) < 1e-12
+    assert np.array_equal(st(s32), st(qb).astype(np.float32))
+    from actx_dgfem_suite_amd.timestepping import LSERK4
+    fused, plain = LSERK4(plan, dt, precision, fused=True), LSERK4(plan, dt, precision, fused=False)
+    assert fused.fused and not plain.fused
+    q1 = [torch.from_numpy(f.copy()).to(device) for f in q0]
+    q2 = [t.clone() for t in q1]
+    fused.run(q1, 2)
+    plain.run(q2, 2)
+    torch.cuda.synchronize()
+    assert rel_l2(st(q1), st(q2)) < 1e-14
+
+
+def test_fused_mixed_stage_rejects_misuse(device):
+    d, _ = discr_pair(2, 2)
+    plan = DGPlan(d, "wave", 1.0)
+    from actx_dgfem_suite_amd.timestepping import LSERK4
+    # defaults follow the measured cases: wave P2 fused in BF16x3, not FP32
+    assert LSERK4(plan, 1e-3, "bf16x3").fused and not LSERK4(plan, 1e-3, "fp32").fused
+    q, res = plan.empty_fields(), plan.empty_fields()
+    s32, d32 = plan.empty_fields(torch.float32), plan.empty_fields(torch.float32)
+    with pytest.raises(ValueError, match="q32_out must not overlap q32_in"):
+        plan.rhs_lserk_stage_mixed("fp32", 0.0, 0.5, 1e-3, s32, s32, q, res)
+    with pytest.raises(ValueError, match="res must not overlap q "):
+        plan.rhs_lserk_stage_mixed("fp32", 0.0, 0.5, 1e-3, s32, d32, q, q)
+    # an FP32 shadow placed inside the FP64 state's memory
+    inside = [q[0].view(torch.float32).view(-1)[:q[0].numel()].view(plan.shape), *d32[1:]]
+    with pytest.raises(ValueError, match="q32_out must not overlap q"):
+        plan.rhs_lserk_stage_mixed("fp32", 0.0, 0.5, 1e-3, s32, inside, q, res)
+    with pytest.raises(NotImplementedError):
+        plan.rhs_lserk_stage_mixed("fp16", 0.0, 0.5, 1e-3, s32, d32, q, res)
+    d2, _ = discr_pair_2d(2, 2)
+    tri = DGPlan(d2, "wave", 1.0)
+    assert not tri.fused_stage_mixed
+    with pytest.raises(NotImplementedError, match="tetrahedral"):
+        tri.rhs_lserk_stage_mixed("fp32", 0.0, 0.5, 1e-3, tri.empty_fields(torch.float32),
+                                  tri.empty_fields(torch.float32), tri.empty_fields(),
+                                  tri.empty_fields())
 
 
 def test_plan_rejects_bad_fields(device):
