@@ -1506,6 +1506,39 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
   return DG_OK;
 }
 
+// dg_adjoint_space in either storage precision (FP64 arithmetic).
+template <typename T>
+int adjoint_space_impl(const dg_plan* p, int32_t to_adjoint, const T* const* in, T* const* out,
+                       void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->has_vjp)
+    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: plan created without mass_inv / face_mass");
+  if (p->equation == DG_EQ_EULER)
+    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: the Euler operator is not linear");
+  int rc;
+  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
+      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
+    return rc;
+  MassArgs<T> m{};
+  for (int c = 0; c < p->nfields; ++c) {
+    m.in[c] = in[c];
+    m.out[c] = out[c];
+  }
+  adjoint_reflection(p, m.sign);
+  m.matrix = to_adjoint ? p->mass_inv : p->mass;
+  m.jac = p->jac;
+  m.jac_power = to_adjoint ? -1 : 1;
+  m.nelem = p->nelem;
+  MassKernel<T> mk;
+  if constexpr (sizeof(T) == 4)
+    mk = p->mass_kernel32;
+  else
+    mk = p->mass_kernel;
+  hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+  DG_HIP_CHECK(hipGetLastError());
+  return DG_OK;
+}
+
 template <typename T>
 int lserk_adjoint_impl(const dg_plan* p, double a, double b, double dt, T* const* qbar,
                        T* const* sv, const T* const* w, void* stream) {
@@ -1548,28 +1581,12 @@ int dg_rhs_vjp_fp32(const dg_plan* p, const float* const* q, const float* const*
 
 int dg_adjoint_space(const dg_plan* p, int32_t to_adjoint, const double* const* in,
                      double* const* out, void* stream) {
-  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
-  if (!p->has_vjp)
-    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: plan created without mass_inv / face_mass");
-  if (p->equation == DG_EQ_EULER)
-    return fail(DG_ERR_UNSUPPORTED, "dg_adjoint_space: the Euler operator is not linear");
-  int rc;
-  if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
-      (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")))
-    return rc;
-  MassArgs<double> m{};
-  for (int c = 0; c < p->nfields; ++c) {
-    m.in[c] = in[c];
-    m.out[c] = out[c];
-  }
-  adjoint_reflection(p, m.sign);
-  m.matrix = to_adjoint ? p->mass_inv : p->mass;
-  m.jac = p->jac;
-  m.jac_power = to_adjoint ? -1 : 1;
-  m.nelem = p->nelem;
-  hipLaunchKernelGGL(p->mass_kernel, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
-  DG_HIP_CHECK(hipGetLastError());
-  return DG_OK;
+  return adjoint_space_impl<double>(p, to_adjoint, in, out, stream);
+}
+
+int dg_adjoint_space_fp32(const dg_plan* p, int32_t to_adjoint, const float* const* in,
+                          float* const* out, void* stream) {
+  return adjoint_space_impl<float>(p, to_adjoint, in, out, stream);
 }
 
 int dg_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt, double* const* qbar,

@@ -314,19 +314,24 @@ class DGPlan:
                       out: Sequence[torch.Tensor] | None = None) -> list[torch.Tensor]:
         """``R M^{-1} fields`` (``to_adjoint``) or ``R M fields``: the change of
         variables in which the linear operators' transpose is the forward RHS
-        (``dg_adjoint_space``; float64 wave and Maxwell plans).  ``out`` may be
-        ``fields`` itself."""
-        self._check_fields(fields, "fields", torch.float64)
+        (``dg_adjoint_space``, or ``dg_adjoint_space_fp32`` for float32
+        fields; wave and Maxwell plans).  ``out`` may be ``fields`` itself."""
+        dtype = fields[0].dtype if len(fields) else torch.float64
+        if dtype not in (torch.float64, torch.float32):
+            raise TypeError(f"adjoint_space: float64 or float32 fields, got {dtype}")
+        self._check_fields(fields, "fields", dtype)
         if out is None:
-            out = self.empty_fields()
+            out = self.empty_fields(dtype)
         else:
-            self._check_fields(out, "out", torch.float64)
+            self._check_fields(out, "out", dtype)
+        fn, name = ((self.lib.dg_adjoint_space, "dg_adjoint_space") if dtype == torch.float64
+                    else (self.lib.dg_adjoint_space_fp32, "dg_adjoint_space_fp32"))
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        _lib.check(self.lib, self.lib.dg_adjoint_space(
+        _lib.check(self.lib, fn(
             self._handle, 1 if to_adjoint else 0,
             _lib.pointer_array([t.data_ptr() for t in fields]),
             _lib.pointer_array([t.data_ptr() for t in out]),
-            ctypes.c_void_p(stream)), "dg_adjoint_space")
+            ctypes.c_void_p(stream)), name)
         return list(out)
 
     def close(self) -> None:

@@ -93,8 +93,11 @@ class LSERK4:
     ``precision`` is the forward RHS: ``"fp64"``, or the mixed ``"fp32"`` /
     ``"bf16x3"`` forms, which keep the state and residual in FP64 and feed the
     RHS an FP32 shadow of the state (``dg_lserk_stage_mixed``).  The reverse
-    step runs in the dtype of the cotangent it is given: float64
-    (``dg_rhs_vjp``) or float32 (the FP32 adjoint, ``dg_rhs_vjp_fp32``).
+    step runs in the dtype of the cotangent it is given: float64 or float32.
+    The linear operators' reverse steps run in the variables of
+    ``plan.adjoint_space`` on the forward RHS (FP64, or the FP32 mixed
+    kernel); Euler's on the VJP (``dg_rhs_vjp``, or the FP32 adjoint
+    ``dg_rhs_vjp_fp32``).
     """
 
     def __init__(self, plan, dt: float, precision: str = "fp64", fused: bool | None = None):
@@ -198,22 +201,26 @@ class LSERK4:
         return buf
 
     def _adjoint_space_sweep(self, q_hat: Sequence[torch.Tensor]) -> bool:
-        """Whether a float64 cotangent of this stepper is swept in the
-        variables of ``plan.adjoint_space`` (linear operators only)."""
-        return self.linear and q_hat[0].dtype == torch.float64
+        """Whether a cotangent of this stepper is swept in the variables of
+        ``plan.adjoint_space``: linear operators, float64 (on the FP64 RHS)
+        or float32 (on the FP32 mixed RHS)."""
+        return self.linear and q_hat[0].dtype in (torch.float64, torch.float32)
 
     def step_vjp_adjoint_space(self, q_hat: Sequence[torch.Tensor]) -> None:
-        """One reverse step of a linear operator on a float64 cotangent held
-        in the variables ``q_hat = R M^{-1} q_bar`` of ``plan.adjoint_space``.
-        There the transposed RHS is the forward RHS (``L^T = R M L R
-        M^{-1}``), so each reverse stage is one ``dg_rhs`` and one
-        ``dg_lserk_stage_adjoint``: the reverse step costs what a forward
-        step does."""
+        """One reverse step of a linear operator on a cotangent held in the
+        variables ``q_hat = R M^{-1} q_bar`` of ``plan.adjoint_space``.  There
+        the transposed RHS is the forward RHS (``L^T = R M L R M^{-1}``), so
+        each reverse stage is one forward RHS and one reverse stage update
+        (float64: ``dg_rhs``, ``dg_lserk_stage_adjoint``; float32: the FP32
+        mixed ``dg_rhs_mixed``, ``dg_lserk_stage_adjoint_fp32``): the reverse
+        step costs what a forward step does."""
         plan = self.plan
-        s, w = self._adjoint_buffers(torch.float64)
+        dtype = q_hat[0].dtype
+        s, w = self._adjoint_buffers(dtype)
+        precision = "fp64" if dtype == torch.float64 else "fp32"
         plan.lserk_stage_adjoint(0.0, LSERK4_B[4], self.dt, q_hat, s, None)
         for k in range(4, -1, -1):
-            plan.rhs(s, w)
+            plan.rhs(s, w, precision=precision)
             a = LSERK4_A[k]
             b = LSERK4_B[k - 1] if k > 0 else 0.0
             plan.lserk_stage_adjoint(a, b, self.dt, q_hat, s, w)

@@ -248,6 +248,13 @@ int dg_lserk_stage_adjoint_fp32(const dg_plan* plan, double a, double b, double 
 int dg_adjoint_space(const dg_plan* plan, int32_t to_adjoint, const double* const* in,
                      double* const* out, void* stream);
 
+/* dg_adjoint_space on float32 fields (FP64 arithmetic).  An FP32 reverse
+ * sweep runs in these variables on dg_rhs_mixed (DG_PREC_FP32) and
+ * dg_lserk_stage_adjoint_fp32.  No counterpart in the reference (see
+ * dg_rhs_vjp). */
+int dg_adjoint_space_fp32(const dg_plan* plan, int32_t to_adjoint, const float* const* in,
+                          float* const* out, void* stream);
+
 /* Message for the last failing call on this thread ("" if none). */
 const char* dg_last_error_string(void);
 
