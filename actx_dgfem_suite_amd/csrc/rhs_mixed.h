@@ -174,6 +174,18 @@ constexpr bool mixed_tile_tables() {
   return ORDER <= 2;
 }
 
+// Whether the stage epilogue issues a field's res and q loads before the
+// tile goes through LDS (all in flight together) or loads them in its store
+// loop.  Preloading costs 2 NSLOT registers per field; in-loop where that
+// would cost a wave per SIMD (FP32 wave P1-P2, Euler P2-P3; BF16x3 Maxwell
+// P2).
+template <int EQ, int PREC, int ORDER>
+constexpr bool mixed_stage_preload() {
+  if constexpr (PREC == DG_PREC_FP32)
+    return !((EQ == DG_EQ_WAVE && ORDER <= 2) || (EQ == DG_EQ_EULER && (ORDER == 2 || ORDER == 3)));
+  return !(EQ == DG_EQ_MAXWELL && ORDER == 2);
+}
+
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
 // res = a res + dt rhs, q = q + b res, q32 = (float)q to it in its output
 // phase (the FP32 RHS tile is the one the unfused path would store).
@@ -501,12 +513,13 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 
     // STAGE: each lane's slots of the output tile, idx = lane + 64 t
     constexpr int NSLOT = (16 * NP + 63) / 64;
+    constexpr bool PRE = STAGE && mixed_stage_preload<EQ, PREC, ORDER>();
 #pragma unroll
     for (int c = 0; c < NF; ++c) {
       // STAGE: this field's res and q loads are issued before the tile goes
       // through LDS, so they are in flight together
-      double st_res[STAGE ? NSLOT : 1], st_q[STAGE ? NSLOT : 1];
-      if constexpr (STAGE) {
+      double st_res[PRE ? NSLOT : 1], st_q[PRE ? NSLOT : 1];
+      if constexpr (PRE) {
         const int64_t off = tile * 16 * NP;
 #pragma unroll
         for (int t = 0; t < NSLOT; ++t) {
@@ -531,8 +544,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         for (int t = 0; t < NSLOT; ++t) {
           const int idx = lane + 64 * t;
           if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem) {
-            const double r = (a.st_a != 0.0 ? a.st_a * st_res[t] : 0.0) + a.st_dt * (double)buf[idx];
-            const double qn = st_q[t] + a.st_b * r;
+            const double r0 = PRE ? st_res[PRE ? t : 0] : (a.st_a != 0.0 ? a.st_res[c][off + idx] : 0.0);
+            const double q0 = PRE ? st_q[PRE ? t : 0] : a.st_q[c][off + idx];
+            const double r = (a.st_a != 0.0 ? a.st_a * r0 : 0.0) + a.st_dt * (double)buf[idx];
+            const double qn = q0 + a.st_b * r;
             a.st_res[c][off + idx] = r;
             a.st_q[c][off + idx] = qn;
             a.st_q32[c][off + idx] = (float)qn;

@@ -58,12 +58,13 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
 
 # Mixed forms whose fused step measured at least 4% faster on MI355X (10
 # whole steps through ``run`` on meshes of 94k-750k tets, fused against the
-# mixed RHS and stage kernels).  FP32: wave P3 1.08x, P4 1.14x, Euler P1
-# 1.06x.  BF16x3: wave P2 1.08x, P3 1.05x, P4 1.12x, Euler P3 1.12x, Maxwell
-# P3 1.07x, P4 1.06x.  Elsewhere it was slower (0.69x-0.98x; Euler P2 BF16x3
-# 1.02x), mostly where the stage epilogue's registers cost a wave per SIMD.
+# mixed RHS and stage kernels).  FP32: wave P1 1.10x, P3 1.08x, P4 1.14x,
+# Euler P1 1.06x.  BF16x3: wave P2 1.08x, P3 1.05x, P4 1.12x, Euler P3
+# 1.12x, Maxwell P3 1.07x, P4 1.06x.  Elsewhere it was slower (0.65x-0.99x;
+# Euler P2 BF16x3 1.02x), mostly where the stage epilogue's registers cost a
+# wave per SIMD.
 _FUSED_MIXED = {
-    "fp32": {("wave", 3), ("wave", 4), ("euler", 1)},
+    "fp32": {("wave", 1), ("wave", 3), ("wave", 4), ("euler", 1)},
     "bf16x3": {("wave", 2), ("wave", 3), ("wave", 4), ("euler", 3), ("maxwell", 3),
                ("maxwell", 4)},
 }
