@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "dg_rhs_lserk_stage",
     "dg_rhs_lserk_stage_mixed",
     "dg_adjoint_space_fp32",
+    "dg_rhs_lserk_stage_adjoint",
     "dg_lserk_stage_mixed",
     "dg_rhs_vjp",
     "dg_lserk_stage_adjoint",
@@ -119,6 +120,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_adjoint_space_fp32.argtypes = [ctypes.c_void_p, ctypes.c_int32, _PP, _PP,
                                           ctypes.c_void_p]
     lib.dg_adjoint_space_fp32.restype = ctypes.c_int
+    lib.dg_rhs_lserk_stage_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_rhs_lserk_stage_adjoint.restype = ctypes.c_int
     lib.dg_last_error_string.argtypes = []
     lib.dg_last_error_string.restype = ctypes.c_char_p
     return lib

@@ -59,10 +59,12 @@ struct RhsArgs {
   double post_sign[kMaxFields];
   // MFMA kernel with the LSERK4 stage fused into its output (STAGE), from
   // the input state q: res = a * res + dt * rhs (res not read when a == 0);
-  // q_out = q + b * res.  q_out must not be q (other tiles gather from q).
+  // q_out = beta * q + b * res.  q_out must not be q (other tiles gather
+  // from q).  Forward stage: beta = 1; reverse stage in the adjoint
+  // variables: res is q_hat, a = 1, q is s.
   OutPtrs stage_res;
   OutPtrs stage_qout;
-  double stage_a, stage_b, stage_dt;
+  double stage_a, stage_b, stage_dt, stage_beta;
   int64_t nelem;
   double p0;
 };

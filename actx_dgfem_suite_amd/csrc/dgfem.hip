@@ -1323,38 +1323,57 @@ int dg_rhs(const dg_plan* p, int32_t precision, const double* const* in, double*
   return DG_OK;
 }
 
-int dg_rhs_lserk_stage(const dg_plan* p, double a, double b, double dt, const double* const* q_in,
-                       double* const* q_out, double* const* res, void* stream) {
+// The MFMA kernel with the stage fused: res = alpha res + dt L(in);
+// out = beta in + b res.  names: in, out, res, entry point.
+static int fused_stage_impl(const dg_plan* p, double alpha, double beta, double b, double dt,
+                            const double* const* in, double* const* out, double* const* res,
+                            void* stream, const char* const names[4]) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
   if (!p->stage_kernel)
-    return fail(DG_ERR_UNSUPPORTED,
-                "dg_rhs_lserk_stage: tetrahedral plans on the MFMA kernel only (use dg_rhs and "
-                "dg_lserk_stage)");
+    return fail(DG_ERR_UNSUPPORTED, std::string(names[3]) +
+                                        ": tetrahedral plans on the MFMA kernel only (use dg_rhs "
+                                        "and the stage kernel)");
   int rc;
-  if ((rc = check_ptrs((const void* const*)q_in, p->nfields, "q_in")) ||
-      (rc = check_ptrs((const void* const*)q_out, p->nfields, "q_out")) ||
-      (rc = check_ptrs((const void* const*)res, p->nfields, "res")))
+  if ((rc = check_ptrs((const void* const*)in, p->nfields, names[0])) ||
+      (rc = check_ptrs((const void* const*)out, p->nfields, names[1])) ||
+      (rc = check_ptrs((const void* const*)res, p->nfields, names[2])))
     return rc;
   const size_t bytes = (size_t)(p->nelem * p->np) * sizeof(double);
-  if ((rc = check_disjoint((const void* const*)q_out, "q_out", bytes, (const void* const*)q_in,
-                           "q_in", bytes, p->nfields)) ||
-      (rc = check_disjoint((const void* const*)res, "res", bytes, (const void* const*)q_in, "q_in",
-                           bytes, p->nfields)) ||
-      (rc = check_disjoint((const void* const*)res, "res", bytes, (const void* const*)q_out,
-                           "q_out", bytes, p->nfields)))
+  if ((rc = check_disjoint((const void* const*)out, names[1], bytes, (const void* const*)in,
+                           names[0], bytes, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, names[2], bytes, (const void* const*)in,
+                           names[0], bytes, p->nfields)) ||
+      (rc = check_disjoint((const void* const*)res, names[2], bytes, (const void* const*)out,
+                           names[1], bytes, p->nfields)))
     return rc;
   double* none[kMaxFields] = {};
-  RhsArgs r = rhs_args(p, q_in, none, false);
+  RhsArgs r = rhs_args(p, in, none, false);
   for (int c = 0; c < p->nfields; ++c) {
     r.stage_res.p[c] = res[c];
-    r.stage_qout.p[c] = q_out[c];
+    r.stage_qout.p[c] = out[c];
   }
-  r.stage_a = a;
+  r.stage_a = alpha;
   r.stage_b = b;
   r.stage_dt = dt;
+  r.stage_beta = beta;
   hipLaunchKernelGGL(p->stage_kernel, dim3(p->stage_grid), dim3(kBlock), 0, (hipStream_t)stream, r);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
+}
+
+int dg_rhs_lserk_stage(const dg_plan* p, double a, double b, double dt, const double* const* q_in,
+                       double* const* q_out, double* const* res, void* stream) {
+  static const char* const names[4] = {"q_in", "q_out", "res", "dg_rhs_lserk_stage"};
+  return fused_stage_impl(p, a, 1.0, b, dt, q_in, q_out, res, stream, names);
+}
+
+int dg_rhs_lserk_stage_adjoint(const dg_plan* p, double a, double b, double dt,
+                               const double* const* s_in, double* const* s_out,
+                               double* const* q_hat, void* stream) {
+  static const char* const names[4] = {"s_in", "s_out", "q_hat", "dg_rhs_lserk_stage_adjoint"};
+  if (p && p->equation == DG_EQ_EULER)
+    return fail(DG_ERR_UNSUPPORTED, "dg_rhs_lserk_stage_adjoint: the Euler operator is not linear");
+  return fused_stage_impl(p, 1.0, a, b, dt, s_in, s_out, q_hat, stream, names);
 }
 
 int dg_lserk_stage(const dg_plan* p, double a, double b, double dt, double* const* q,

@@ -183,8 +183,9 @@ __device__ constexpr int levi(int o, int x, int a) {
 // the B operand -- the last step of the linear operators' adjoint
 // R M L R M^{-1} lam, fused so that it costs no pass over HBM.
 // STAGE: instead of storing the RHS, the kernel applies the LSERK4 stage
-// res = a res + dt rhs, q_out = q + b res to it in its output phase, so the
-// RHS makes no round trip through HBM between the two.
+// res = a res + dt rhs, q_out = beta q + b res to it in its output phase, so
+// the RHS makes no round trip through HBM between the two (beta = 1 forward;
+// the reverse stage in the adjoint variables is res = q_hat, a = 1).
 template <int EQ, int ORDER, bool POST = false, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   using S = MfmaShape<ORDER>;
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem) {
             const double r = (a.stage_a != 0.0 ? a.stage_a * st_res[t] : 0.0) + a.stage_dt * buf[idx];
             res[idx] = r;
-            qo[idx] = st_q[t] + a.stage_b * r;
+            qo[idx] = a.stage_beta * st_q[t] + a.stage_b * r;
           }
         }
       } else {

@@ -224,6 +224,22 @@ class DGPlan:
             _lib.pointer_array([t.data_ptr() for t in res]),
             ctypes.c_void_p(stream)), "dg_rhs_lserk_stage")
 
+    def rhs_lserk_stage_adjoint(self, a: float, b: float, dt: float, s_in: Sequence[torch.Tensor],
+                                s_out: Sequence[torch.Tensor], q_hat: Sequence[torch.Tensor]) -> None:
+        """The reverse LSERK4 stage of a linear operator in the variables of
+        ``adjoint_space``, in one kernel (``dg_rhs_lserk_stage_adjoint``):
+        ``w = L(s_in); q_hat += dt*w; s_out = a*s_in + b*q_hat`` (float64;
+        plans with ``fused_stage``)."""
+        for name, f in (("s_in", s_in), ("s_out", s_out), ("q_hat", q_hat)):
+            self._check_fields(f, name)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib, self.lib.dg_rhs_lserk_stage_adjoint(
+            self._handle, float(a), float(b), float(dt),
+            _lib.pointer_array([t.data_ptr() for t in s_in]),
+            _lib.pointer_array([t.data_ptr() for t in s_out]),
+            _lib.pointer_array([t.data_ptr() for t in q_hat]),
+            ctypes.c_void_p(stream)), "dg_rhs_lserk_stage_adjoint")
+
     @property
     def fused_stage_mixed(self) -> bool:
         """Whether ``rhs_lserk_stage_mixed`` is available: tetrahedral plans."""

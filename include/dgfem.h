@@ -170,6 +170,19 @@ int dg_rhs_lserk_stage(const dg_plan* plan, double a, double b, double dt,
                        const double* const* q_in, double* const* q_out,
                        double* const* res, void* stream);
 
+/* The reverse LSERK4 stage of a linear operator (wave, Maxwell) in the
+ * variables of dg_adjoint_space, where the transposed RHS is the RHS:
+ *   w = L(s_in);  q_hat = q_hat + dt*w;  s_out = a*s_in + b*q_hat
+ * in one kernel (dg_rhs then dg_lserk_stage_adjoint, with the update applied
+ * in the RHS kernel's output phase).  q_hat is updated in place; s_out and
+ * q_hat must share no memory with s_in or each other
+ * (DG_ERR_INVALID_ARGUMENT otherwise).  Tetrahedral plans on the MFMA kernel
+ * only; others, and Euler plans, return DG_ERR_UNSUPPORTED.  No counterpart
+ * in the reference (see dg_rhs_vjp). */
+int dg_rhs_lserk_stage_adjoint(const dg_plan* plan, double a, double b, double dt,
+                               const double* const* s_in, double* const* s_out,
+                               double* const* q_hat, void* stream);
+
 /* LSERK4 stage for the mixed form: FP64 q and res, FP32 rhs in, and the FP32
  * shadow q32 = (float)q written for the next mixed RHS evaluation.  No
  * counterpart in the reference (see dg_lserk_stage). */

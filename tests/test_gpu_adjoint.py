@@ -513,6 +513,40 @@ def test_fp32_adjoint_space_sweep_matches_fp64(dim, equation, device):
     assert e < 1e-5
 
 
+@pytest.mark.parametrize("equation", ["wave", "maxwell"])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_fused_reverse_stage_matches_unfused(equation, order, device):
+    """dg_rhs_lserk_stage_adjoint (the reverse stage update applied in the
+    MFMA kernel's output phase) against dg_rhs followed by
+    dg_lserk_stage_adjoint: two reverse steps in the adjoint variables on a
+    mesh with a partial last tile."""
+    print(f"[case] fused reverse stage {equation} P{order} n=3", flush=True)
+    d, _ = discr_pair(order, 3)
+    plan = DGPlan(d, equation, PARAM[equation], variant="mfma")
+    fused, plain = LSERK4(plan, 1e-3, fused=True), LSERK4(plan, 1e-3, fused=False)
+    assert fused._fused_adjoint and not plain._fused_adjoint
+    y = _dev(np.random.default_rng(order).standard_normal((plan.nfields, d.nelements, d.np)),
+             device)
+    a, b = [t.clone() for t in y], [t.clone() for t in y]
+    for _ in range(2):
+        fused.step_vjp_adjoint_space(a)
+        plain.step_vjp_adjoint_space(b)
+    e = max(float(torch.linalg.norm(x - z) / torch.linalg.norm(z)) for x, z in zip(a, b))
+    print(f"  rel err fused vs unfused {e:.2e}", flush=True)
+    assert e < 1e-14
+    s = plan.empty_fields()
+    with pytest.raises(ValueError, match="s_out must not overlap s_in"):
+        plan.rhs_lserk_stage_adjoint(0.5, 0.5, 1e-3, s, s, a)
+    with pytest.raises(ValueError, match="q_hat must not overlap s_in"):
+        plan.rhs_lserk_stage_adjoint(0.5, 0.5, 1e-3, s, plan.empty_fields(), s)
+    if order == 1:
+        de, _ = discr_pair(1, 2)
+        eu = DGPlan(de, "euler", PARAM["euler"], variant="mfma")
+        f = eu.empty_fields()
+        with pytest.raises(NotImplementedError, match="not linear"):
+            eu.rhs_lserk_stage_adjoint(0.5, 0.5, 1e-3, f, eu.empty_fields(), eu.empty_fields())
+
+
 def test_fp32_euler_vjp_matches_fp64(device):
     d, od = discr_pair(2, 4)
     plan = DGPlan(d, "euler", PARAM["euler"])
