@@ -38,7 +38,7 @@ class DGOperator:
     """The operator an ``rhs`` callable evaluates.
 
     ``takes_time``: the callable's signature is ``rhs(t, state)`` (Euler,
-    Maxwell: equations/euler.py:176, equations/maxwell.py:264) rather than
+    Maxwell: equations/euler.py:176, equations/maxwell.py:69-70) rather than
     ``rhs(state)`` (wave: equations/wave.py:281).  The time argument is
     unused by all three operators, as in the reference.
     """

@@ -7,7 +7,7 @@ order the C ABI uses.
   ``grudge.models.euler.ConservedEulerField`` as used in
   equations/euler.py:37,91-93.
 * Maxwell state is a flat object array ``[E_x, E_y, E_z, H_x, H_y, H_z]``
-  (equations/maxwell.py:256-269).
+  (equations/maxwell.py:61-65, 73-74).
 
 Leaves are ``(K, Np)`` arrays (torch tensors on the device, numpy on the host).
 """

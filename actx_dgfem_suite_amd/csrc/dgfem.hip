@@ -1448,6 +1448,33 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
       (rc = check_ptrs((const void* const*)q_bar, p->nfields, "q_bar")) ||
       (rc = check_ptrs((const void* const*)workspace, p->nfields, "workspace")))
     return rc;
+  // The second launch gathers neighbour values of the workspace (and, for
+  // Euler, of q) while it writes q_bar, and the first writes the workspace
+  // before q is read.  The mass pass into the workspace is safe in place on
+  // the same field, so workspace[c] may be rhs_bar[c] but overlap nothing
+  // else of rhs_bar.
+  const size_t fbytes = (size_t)(p->nelem * p->np) * sizeof(T);
+  if ((rc = check_disjoint((const void* const*)q_bar, "q_bar", fbytes,
+                           (const void* const*)workspace, "workspace", fbytes, p->nfields)))
+    return rc;
+  if (p->equation == DG_EQ_EULER &&
+      ((rc = check_disjoint((const void* const*)q_bar, "q_bar", fbytes, (const void* const*)q, "q",
+                            fbytes, p->nfields)) ||
+       (rc = check_disjoint((const void* const*)workspace, "workspace", fbytes,
+                            (const void* const*)q, "q", fbytes, p->nfields))))
+    return rc;
+  for (int c = 0; c < p->nfields; ++c)
+    for (int c2 = 0; c2 < p->nfields; ++c2) {
+      if (!(c == c2 && (const void*)workspace[c] == (const void*)rhs_bar[c]) &&
+          overlaps(workspace[c], fbytes, rhs_bar[c2], fbytes))
+        return fail(DG_ERR_INVALID_ARGUMENT,
+                    std::string(name) + ": workspace[" + std::to_string(c) +
+                        "] overlaps rhs_bar[" + std::to_string(c2) + "]");
+      if (c2 > c && overlaps(workspace[c], fbytes, workspace[c2], fbytes))
+        return fail(DG_ERR_INVALID_ARGUMENT,
+                    std::string(name) + ": workspace[" + std::to_string(c) + "] and [" +
+                        std::to_string(c2) + "] overlap");
+    }
   constexpr bool f32 = sizeof(T) == 4;
   MassArgs<T> m{};
   for (int c = 0; c < p->nfields; ++c) {

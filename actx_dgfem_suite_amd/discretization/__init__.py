@@ -3,7 +3,7 @@ connectivity, packed into the arrays the device plan consumes.
 
 Stands in for grudge's ``DiscretizationCollection`` built in
 get_{wave,euler,maxwell}_rhs (equations/wave.py:261-270,
-equations/euler.py:157-166, equations/maxwell.py:250).
+equations/euler.py:157-166, equations/maxwell.py:55).
 """
 
 from __future__ import annotations

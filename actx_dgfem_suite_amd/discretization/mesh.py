@@ -3,7 +3,7 @@
 Host-side setup that stands in for ``meshmode.mesh.generation.
 generate_regular_rect_mesh`` and the geometry grudge derives from it
 (equations/wave.py:248-270, equations/euler.py:153-166,
-equations/maxwell.py:246-250): a box split into ``n^3`` hexahedra, each cut
+equations/maxwell.py:50-53): a box split into ``n^3`` hexahedra, each cut
 into 6 tetrahedra, with affine (per-element constant) inverse Jacobians,
 outward unit normals and face area elements.
 

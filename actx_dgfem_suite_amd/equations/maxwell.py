@@ -1,8 +1,8 @@
 """Maxwell operator: builder mirroring ``get_maxwell_rhs``
-(equations/maxwell.py:238-269 of the reference).
+(equations/maxwell.py:43-74 of the reference).
 
 The operator is upstream ``grudge.models.em.MaxwellOperator(epsilon=1, mu=1,
-flux_type=0.5)`` with PEC walls (equations/maxwell.py:252-254):
+flux_type=0.5)`` with PEC walls (equations/maxwell.py:57-59):
 ``w_t = (curl H, -curl E) - M^{-1} face_mass(flux)`` with the strong-form
 local curl and the partially upwind flux of Hesthaven & Warburton p. 433.
 """
@@ -19,7 +19,7 @@ from actx_dgfem_suite_amd.equations._common import box_discretization, make_rhs
 def rectangular_cavity_mode(nodes: np.ndarray, t: float = 0.0, mode=(1, 2, 2)):
     """Real part of the TM cavity mode of the unit cube (the state the
     reference takes from grudge's ``get_rectangular_cavity_mode``,
-    equations/maxwell.py:256-268)."""
+    equations/maxwell.py:61-65)."""
     kx, ky, kz = (m * np.pi for m in mode)
     omega = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
     x, y, z = nodes

@@ -31,7 +31,7 @@ class DGPlan:
 
     ``param`` is the wave speed c (wave), gamma (Euler) or the flux_type alpha
     (Maxwell), matching equations/wave.py:279, equations/euler.py:151 and
-    equations/maxwell.py:252-254.  ``variant`` picks the FP64 kernel:
+    equations/maxwell.py:57-59.  ``variant`` picks the FP64 kernel:
     ``"auto"``/``"mfma"`` (matrix cores) or ``"lds"`` (the baseline).
     Triangle discretizations (``discr.dim == 2``) run the LDS-staged 2D kernel
     (FP64 arithmetic; the mixed precisions store FP32 fields) and its adjoint.

@@ -120,6 +120,7 @@ class LSERK4:
         self.dt = float(dt)
         self.precision = precision
         self._res = plan.empty_fields()
+        explicit = fused
         if fused is None:
             fused = fused_stage_preferred(plan, precision)
         # fused: the stage runs inside the RHS kernel, which reads one state
@@ -129,9 +130,9 @@ class LSERK4:
         # float64 reverse steps of the linear operators in the adjoint
         # variables: fused on an FP64 stepper's explicit choice, else where
         # it measured faster
-        fwd64 = fused if precision == "fp64" else None
+        fwd64 = explicit if precision == "fp64" else None
         self._fused_adjoint = (plan.equation != "euler" and bool(plan.fused_stage) and
-                               (fwd64 if fwd64 is not None else
+                               (bool(fwd64) if fwd64 is not None else
                                 (plan.equation, plan.discr.order) in _FUSED_REVERSE))
         if precision == "fp64":
             self._rhs = None if self._q2 is not None else plan.empty_fields()

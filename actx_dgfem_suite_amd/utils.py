@@ -33,11 +33,26 @@ def get_nel_1d_for_regular_rect_mesh(dim: int, order: int, ndofs: int) -> int:
 
 
 def get_ndof_for_regular_rect_mesh(dim: int, order: int, nel_1d: int) -> int:
-    """Nodes per field of the box mesh.
+    """DOFs per field as the reference counts them (utils.py:46-74).
 
-    The reference (utils.py:49-50) multiplies by 6 nodes for 3D P1, which is
-    the tet count per hexahedron, not Np; this uses Np = 4 for P1.
-    """
+    Bit-for-bit the reference's integer, including its 3D P1 case, which
+    multiplies the 6*n^3 tets by 6 (the tets per hexahedron) where a P1 tet
+    has 4 nodes.  The reference's DOF-scaling tool divides by this count
+    (tools/wave_dof_scaling.py:158-167), so GDOF/s figures comparable with
+    it use it; :func:`count_nodes_regular_rect_mesh` is the true count."""
+    if dim == 3:
+        if order not in _NP_3D:
+            raise NotImplementedError(order)
+        return 6 * nel_1d ** 3 * (6 if order == 1 else _NP_3D[order])
+    if dim == 2:
+        if order not in _NP_2D:
+            raise NotImplementedError(order)
+        return 2 * nel_1d ** 2 * _NP_2D[order]
+    raise NotImplementedError
+
+
+def count_nodes_regular_rect_mesh(dim: int, order: int, nel_1d: int) -> int:
+    """True nodes per field of the box mesh: elements times Np."""
     if dim == 3:
         if order not in _NP_3D:
             raise NotImplementedError(order)

@@ -6,8 +6,10 @@ flux; 1000 timed RHS evaluations.  A "step" is one RHS evaluation
 ``rhs = L(q)`` with ``q`` already resident in HBM.
 
 Prints one JSON line with the metric, the roofline position of the fused RHS
-kernel (timed with HIP events around every launch on the launch stream) and a
-CPU baseline (the numpy oracle on a bounded sample of the same workload).
+kernel and a CPU baseline (the numpy oracle on a bounded sample of the same
+workload).  The kernel time is one HIP event pair on the launch stream
+around all K back-to-back launches, divided by K: it includes the gaps
+between consecutive launches.
 
     python bench.py [--steps 1000] [--warmup 20] [--workload wave_p3]
 """

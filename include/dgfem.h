@@ -5,7 +5,7 @@
  * `compiled_rhs = actx.compile(rhs); compiled_rhs(*args)` (run/__main__.py:74-76,
  * :109, :126), where `rhs` is the wave / Euler / Maxwell operator built by
  * get_{wave,euler,maxwell}_rhs (equations/wave.py:239-285,
- * equations/euler.py:140-186, equations/maxwell.py:238-269) and the array
+ * equations/euler.py:140-186, equations/maxwell.py:43-74) and the array
  * context turns the operator's einsum DAG into device kernels
  * (arraycontext/dgfem_optimizer.py:181-245).  This library is what a new array
  * context binds to in place of that code generator.  Each entry point below
@@ -44,7 +44,7 @@ typedef enum dg_status {
 typedef enum dg_equation {
   DG_EQ_WAVE = 0,     /* equations/wave.py:121-200 */
   DG_EQ_EULER = 1,    /* equations/euler.py:168-179 (EulerOperator, LF, walls) */
-  DG_EQ_MAXWELL = 2   /* equations/maxwell.py:252-265 (MaxwellOperator, PEC) */
+  DG_EQ_MAXWELL = 2   /* equations/maxwell.py:57-59 (MaxwellOperator, PEC) */
 } dg_equation;
 
 typedef enum dg_kernel_variant {
@@ -85,7 +85,7 @@ typedef struct dg_plan_desc {
   const int32_t* ext_map;
   /* wave: params[0] = c; Euler: params[0] = gamma; Maxwell: params[0] =
    * flux_type alpha (0 central .. 1 upwind), with epsilon = mu = 1 as in
-   * equations/maxwell.py:252-254 */
+   * equations/maxwell.py:57-59 */
   double params[4];
   int32_t variant;           /* dg_kernel_variant (triangles: AUTO or LDS) */
   int32_t dim;               /* 3 (or 0): tetrahedra; 2: triangles, which run
@@ -124,7 +124,7 @@ int dg_plan_variant(const dg_plan* plan, int32_t* variant);
 /* One right-hand-side evaluation, rhs = L(q): the call `compiled_rhs(*args)`
  * (run/__main__.py:76,109,126) of an array context, i.e. wave_operator
  * (equations/wave.py:121-200), EulerOperator.operator (equations/euler.py:
- * 176-179) or MaxwellOperator.operator (equations/maxwell.py:264-265).
+ * 176-179) or MaxwellOperator.operator (equations/maxwell.py:69-70).
  * The kernels gather neighbour traces of fields_in while they write
  * fields_out, so an output field that shares memory with an input field or
  * with another output field returns DG_ERR_INVALID_ARGUMENT (also for

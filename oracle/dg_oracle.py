@@ -13,7 +13,7 @@ the operators the reference benchmarks:
 * Euler: upstream ``grudge.models.euler.EulerOperator(flux_type="lf")`` with
   ``InviscidWallBC`` as wired in equations/euler.py:168-179;
 * Maxwell: upstream ``grudge.models.em.MaxwellOperator(epsilon=1, mu=1,
-  flux_type=0.5)`` with PEC walls as wired in equations/maxwell.py:252-265.
+  flux_type=0.5)`` with PEC walls as wired in equations/maxwell.py:57-59.
 
 It follows grudge's *unfused* operator structure -- ``weak_local_div``,
 ``weak_local_grad``, ``local_grad``, ``face_mass`` and ``inverse_mass`` are

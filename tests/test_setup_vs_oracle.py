@@ -12,6 +12,7 @@ from actx_dgfem_suite_amd.discretization.refelem import (
     warp_blend_nodes,
 )
 from actx_dgfem_suite_amd.utils import (
+    count_nodes_regular_rect_mesh,
     get_ndof_for_regular_rect_mesh,
     get_nel_1d_for_regular_rect_mesh,
 )
@@ -109,7 +110,20 @@ def test_mesh_sizing_mirrors_reference():
     assert get_nel_1d_for_regular_rect_mesh(3, 1, 1000) == 4
     assert get_nel_1d_for_regular_rect_mesh(3, 3, 4_000_000) == 33
     assert get_nel_1d_for_regular_rect_mesh(3, 4, 20_000_000) == 46
-    assert get_ndof_for_regular_rect_mesh(3, 1, 4) == 384 * 4
+    # utils.py:46-74, the integers bit for bit (3D P1 counts 6 per tet there)
+    assert get_ndof_for_regular_rect_mesh(3, 1, 4) == 384 * 6
+    assert get_ndof_for_regular_rect_mesh(3, 2, 4) == 384 * 10
+    assert get_ndof_for_regular_rect_mesh(3, 3, 30) == 162_000 * 20
+    assert get_ndof_for_regular_rect_mesh(3, 4, 46) == 584_016 * 35
+    assert get_ndof_for_regular_rect_mesh(2, 1, 5) == 50 * 3
+    assert get_ndof_for_regular_rect_mesh(2, 4, 5) == 50 * 15
+    # the true node count
+    assert count_nodes_regular_rect_mesh(3, 1, 4) == 384 * 4
+    assert count_nodes_regular_rect_mesh(3, 4, 46) == 584_016 * 35
+    assert count_nodes_regular_rect_mesh(2, 2, 5) == 50 * 6
+    for dim, order in [(3, 5), (2, 0), (4, 1)]:
+        with pytest.raises(NotImplementedError):
+            get_ndof_for_regular_rect_mesh(dim, order, 2)
     with pytest.raises(NotImplementedError):
         get_nel_1d_for_regular_rect_mesh(3, 5, 1000)
 
