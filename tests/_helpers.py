@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import os
+import re
 from functools import lru_cache
 
 import numpy as np
@@ -81,3 +83,21 @@ def random_state(equation: str, x: np.ndarray, seed: int = 0, noise: float = 0.0
     nf = nfields(equation, dim)
     return np.stack([np.sin(x[k % dim] + k) * np.exp(-r2) + 0.1 * rng.standard_normal(shape)
                      for k in range(nf)])
+
+
+STUB_MARKER = "# actx_dgfem_suite/arraycontext/hip_dgfem.py"
+
+
+def stub_namespace() -> dict:
+    """Runs the reference-side ctypes stub printed in INTEGRATION.md, verbatim,
+    against the in-tree libdgfem.so; returns its module namespace."""
+    from actx_dgfem_suite_amd import _lib
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "INTEGRATION.md")).read()
+    blocks = [b for b in re.findall(r"```python\n(.*?)```", text, re.S)
+              if b.startswith(STUB_MARKER)]
+    assert len(blocks) == 1, "INTEGRATION.md must hold exactly one stub block"
+    os.environ["LIBDGFEM"] = _lib.LIB_PATH
+    ns: dict = {}
+    exec(compile(blocks[0], "INTEGRATION.md (hip_dgfem.py stub)", "exec"), ns)
+    return ns
