@@ -57,7 +57,20 @@ def test_full_size_fp64_matches_oracle(name, device):
     del plan
     od = dg_oracle.build_discretization(d.mesh.vertices, d.mesh.elements, d.ref.nodes)
     ref = oracle_rhs(w.equation, od, q, w.param)
-    errs = field_errors(got, ref)
+    if w.equation == "euler":
+        # The uniform x-flow's transverse momenta have an RHS that is zero up
+        # to the rounding of their pressure terms (volume and face parts of
+        # size p / h that cancel): measure each field against the larger of
+        # its RHS and its volume term.
+        flux = dg_oracle.euler_volume_flux(q, w.param)
+        errs = []
+        for c in range(len(q)):
+            vol = dg_oracle.inverse_mass(od, sum(dg_oracle.weak_local_d_dx(od, x, flux[c][x])
+                                                 for x in range(3)))
+            den = max(np.linalg.norm(ref[c]), np.linalg.norm(vol))
+            errs.append(float(np.linalg.norm(got[c] - ref[c]) / den))
+    else:
+        errs = field_errors(got, ref)
     print(f"  rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) < TOL_FP64, errs
 
