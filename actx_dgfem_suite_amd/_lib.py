@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "dg_rhs_vjp",
     "dg_lserk_stage_adjoint",
     "dg_rhs_vjp_fp32",
+    "dg_rhs_vjp_noalloc",
     "dg_lserk_stage_adjoint_fp32",
     "dg_adjoint_space",
     "dg_last_error_string",
@@ -104,6 +105,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.dg_lserk_stage_adjoint.restype = ctypes.c_int
     lib.dg_rhs_vjp_fp32.argtypes = [ctypes.c_void_p, _PP, _PP, _PP, _PP, ctypes.c_void_p]
     lib.dg_rhs_vjp_fp32.restype = ctypes.c_int
+    lib.dg_rhs_vjp_noalloc.argtypes = [ctypes.c_void_p, _PP, _PP, _PP, ctypes.c_void_p]
+    lib.dg_rhs_vjp_noalloc.restype = ctypes.c_int
     lib.dg_lserk_stage_adjoint_fp32.argtypes = [ctypes.c_void_p, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, _PP, _PP,
                                                 _PP, ctypes.c_void_p]

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -35,6 +36,13 @@
 #include "rhs_mfma.h"
 #include "rhs_mixed.h"
 #include "rhs_adjoint.h"
+
+// Workspace of dg_rhs_vjp_noalloc, for callers that cannot allocate: one
+// copy per GPU the library runs on, held until the process ends.  Host code
+// never names it: its device address comes from hipGetSymbolAddress, looked
+// up once per device when a plan with the adjoint is created there.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -54,6 +62,26 @@ int fail(int code, const std::string& msg) {
       return fail(DG_ERR_HIP, std::string(#expr) + ": " +                    \
                                   hipGetErrorString(err_));                  \
   } while (0)
+
+// This device's address of `scratch`, looked up on the first call per
+// device (with that device current) and kept.
+int scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) return fail(DG_ERR_HIP, "scratch: invalid device");
+  if ((size_t)device >= addr.size()) addr.resize((size_t)device + 1, nullptr);
+  if (!addr[(size_t)device]) {
+    void* ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+      return fail(DG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(err));
+    addr[(size_t)device] = ptr;
+  }
+  *out = addr[(size_t)device];
+  return DG_OK;
+}
 
 // {{{ LSERK stage
 
@@ -745,6 +773,7 @@ struct dg_plan {
   double* mass_inv = nullptr;    // (Np, Np)
   double* mass = nullptr;        // (Np, Np) reference mass M
   double* jac = nullptr;         // (K) |J| per element
+  void* scratch = nullptr;       // this device's `scratch` (dg_rhs_vjp_noalloc)
   double* ext_fscale = nullptr;  // (K, F)
   int32_t* node_faces = nullptr; // (Np, 3)
 };
@@ -1267,6 +1296,11 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       int64_t mg = (int64_t)prop.multiProcessorCount * occ;
       p->mass_grid = (int)(mg < vbatch ? mg : vbatch);
     }
+    // set-up of dg_rhs_vjp_noalloc: this device's address of `scratch`
+    if ((rc = scratch_address(p->device, &p->scratch)) != DG_OK) {
+      free_plan(p);
+      return rc;
+    }
     p->has_vjp = true;
   }
   *out = p;
@@ -1623,6 +1657,28 @@ int dg_rhs_vjp(const dg_plan* p, const double* const* q, const double* const* rh
 int dg_rhs_vjp_fp32(const dg_plan* p, const float* const* q, const float* const* rhs_bar,
                     float* const* q_bar, float* const* workspace, void* stream) {
   return rhs_vjp_impl<float>(p, q, rhs_bar, q_bar, workspace, stream, "dg_rhs_vjp_fp32");
+}
+
+int dg_rhs_vjp_noalloc(const dg_plan* p, const double* const* q, const double* const* rhs_bar,
+                       double* const* q_bar, void* stream) {
+  if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
+  if (!p->has_vjp || !p->scratch)
+    return fail(DG_ERR_UNSUPPORTED,
+                "dg_rhs_vjp_noalloc: plan created without mass_inv / face_mass: no adjoint");
+  // one field of the workspace per K*Np doubles, each at a 256-byte boundary
+  const size_t stride = ((size_t)p->nelem * (size_t)p->np * sizeof(double) + 255) / 256 * 256;
+  const size_t need = stride * (size_t)p->nfields;
+  if (need > kScratchBytes)
+    return fail(DG_ERR_INVALID_ARGUMENT,
+                "dg_rhs_vjp_noalloc: the workspace needs " + std::to_string(need) +
+                    " bytes (nfields x K x Np doubles), more than the " +
+                    std::to_string(kScratchBytes) + "-byte scratch; use dg_rhs_vjp");
+  double* ws[kMaxFields];
+  for (int c = 0; c < p->nfields; ++c)
+    ws[c] = (double*)((unsigned char*)p->scratch + (size_t)c * stride);
+  // the mass pass writes every workspace entry before anything reads it, so
+  // nothing in the scratch needs clearing
+  return rhs_vjp_impl<double>(p, q, rhs_bar, q_bar, ws, stream, "dg_rhs_vjp_noalloc");
 }
 
 int dg_adjoint_space(const dg_plan* p, int32_t to_adjoint, const double* const* in,

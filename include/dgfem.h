@@ -233,6 +233,24 @@ int dg_rhs_vjp_fp32(const dg_plan* plan, const float* const* q,
                     const float* const* rhs_bar, float* const* q_bar,
                     float* const* workspace, void* stream);
 
+/* dg_rhs_vjp without a workspace argument and without allocating: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a static device array
+ * of 128 MiB, one per GPU the library runs on, held until the process ends;
+ * plan creation looks up its address on the plan's device.  The call
+ * launches the same kernels as dg_rhs_vjp and calls no hipMalloc, hipFree or
+ * other allocating routine.  Limit: nfields x K x Np doubles (each field
+ * rounded up to 256 bytes) must fit in 128 MiB, else DG_ERR_INVALID_ARGUMENT
+ * (for example wave P3 on 162,000 tets fits; Maxwell P4 on 93,750 tets does
+ * not: use dg_rhs_vjp there).  Every call on one GPU shares that GPU's
+ * array, so calls that overlap in time on one GPU (other streams, other
+ * threads) race; the array keeps the last call's contents, and nothing in it
+ * needs to start at zero.  Aliasing rules and errors as dg_rhs_vjp. */
+int dg_rhs_vjp_noalloc(const dg_plan* plan, const double* const* q,
+                       const double* const* rhs_bar, double* const* q_bar,
+                       void* stream);
+
 /* Reverse sweep of one LSERK4 stage (the transpose of dg_lserk_stage):
  *   q_bar = q_bar + dt*w;   s = a*s + b*q_bar
  * where w = dg_rhs_vjp(q_k, s) of the stage.  w may be NULL (the dt*w term is
