@@ -158,7 +158,8 @@ KernelInfo lds_info() {
 
 template <int EQ, int ORDER>
 KernelInfo mfma_info() {
-  return {rhs_mfma_kernel<EQ, ORDER>, kBlock, 16 * (kBlock / 64)};
+  return {rhs_mfma_kernel<EQ, ORDER>, 64 * mfma_waves_of(EQ, ORDER),
+          16 * mfma_waves_of(EQ, ORDER)};
 }
 
 #define DG_DEFINE_PICK(NAME, INFO)                                           \
@@ -1158,7 +1159,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     std::vector<double> vgeo, fgeo;
     std::vector<int32_t> ext_t;
     build_tile_geometry(k, d->geometry, vgeo, fgeo);
-    const int c = flux_chunk_of(d->order);
+    const int c = flux_chunk_of(d->equation, d->order);
     build_item_map(k, d->order, c, (nfp + c - 1) / c, d->ext_map, ext_t);
     if ((rc = upload(&p->vgeo, vgeo.data(), vgeo.size())) ||
         (rc = upload(&p->fgeo, fgeo.data(), fgeo.size())) ||
@@ -1212,8 +1213,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   if (!tri && p->variant == DG_KERNEL_MFMA) {
     p->stage_kernel = pick_mfma_stage(d->equation, d->order);
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->stage_kernel, kBlock,
-                                                      0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->stage_kernel,
+                                                      p->kernel.threads, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
     int64_t sg = (int64_t)prop.multiProcessorCount * occ;
@@ -1263,12 +1264,12 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       }
       p->post_kernel = pick_mfma_post(d->equation, d->order);
       int occ = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->post_kernel, kBlock,
-                                                        0) != hipSuccess ||
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->post_kernel,
+                                                        p->kernel.threads, 0) != hipSuccess ||
           occ < 1)
         occ = 1;
       int64_t pg = (int64_t)prop.multiProcessorCount * occ;
-      const int64_t tbatch = (k + 16 * (kBlock / 64) - 1) / (16 * (kBlock / 64));
+      const int64_t tbatch = (k + p->kernel.batch_elems - 1) / p->kernel.batch_elems;
       if (pg > tbatch) pg = tbatch;
       p->post_grid = (int)((pg + 7) / 8 * 8);
     }
@@ -1390,7 +1391,8 @@ static int fused_stage_impl(const dg_plan* p, double alpha, double beta, double 
   r.stage_b = b;
   r.stage_dt = dt;
   r.stage_beta = beta;
-  hipLaunchKernelGGL(p->stage_kernel, dim3(p->stage_grid), dim3(kBlock), 0, (hipStream_t)stream, r);
+  hipLaunchKernelGGL(p->stage_kernel, dim3(p->stage_grid), dim3(p->kernel.threads), 0,
+                     (hipStream_t)stream, r);
   DG_HIP_CHECK(hipGetLastError());
   return DG_OK;
 }
@@ -1537,7 +1539,8 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
         r.post_tiles = p->post_tiles;
         r.jac = p->jac;
         for (int c = 0; c < p->nfields; ++c) r.post_sign[c] = m.sign[c];
-        hipLaunchKernelGGL(p->post_kernel, dim3(p->post_grid), dim3(kBlock), 0, (hipStream_t)stream,
+        hipLaunchKernelGGL(p->post_kernel, dim3(p->post_grid), dim3(p->kernel.threads), 0,
+                           (hipStream_t)stream,
                            r);
         DG_HIP_CHECK(hipGetLastError());
         return DG_OK;

@@ -95,21 +95,30 @@ constexpr bool prefetch_ext() {
   return !(EQ == DG_EQ_WAVE && (ORDER == 1 || ORDER == 3));
 }
 
+// Waves per block.  Measured on MI355X at P4, blocks of 8 waves (two waves
+// per SIMD, the 64 KB of operator tiles shared by twice the waves, flux chunks
+// of 3 or 4 k-steps to fit) are slower than 4 waves at one wave per SIMD:
+// wave P4 on 584k tets 949-1042 against 879 us.  Twice the tiles in flight
+// hold about 4.6 MB of state per XCD, more than its 4 MB L2, which the flux
+// producer's own-trace and neighbour reads rely on.
+__host__ __device__ constexpr int mfma_waves_of(int eq, int order) { return 4; }
+
 // LIFT k-steps whose flux one wave stages in LDS at a time (P1, P2: one
-// chunk; P3, P4: 5 nodes).  Host code sizes the flux-item tables with it.
-__host__ __device__ constexpr int flux_chunk_of(int order) {
+// chunk; P3, P4: 5 nodes; 3 at P4 measured 5% slower on wave P4).  Host code
+// sizes the flux-item tables with it.
+__host__ __device__ constexpr int flux_chunk_of(int eq, int order) {
   return nfp_of(order) <= 6 ? nfp_of(order) : 5;
 }
 template <int EQ, int ORDER>
-constexpr int flux_chunk() { return flux_chunk_of(ORDER); }
+constexpr int flux_chunk() { return flux_chunk_of(EQ, ORDER); }
 
 // Flux items of one tile: chunk ch, slot it, lane l handle item it*64 + l of
 // that chunk, i.e. face-pair p = item / C (p = face * 16 + element) at LIFT
 // k-step ch * C + item % C.  ext_t holds the exterior-map entry of every
 // item in this order, so each slot's entries are one contiguous 256-byte load.
-__host__ __device__ constexpr int flux_items_of(int order) {
-  return ((nfp_of(order) + flux_chunk_of(order) - 1) / flux_chunk_of(order)) *
-         flux_chunk_of(order) * 64;
+__host__ __device__ constexpr int flux_items_of(int eq, int order) {
+  return ((nfp_of(order) + flux_chunk_of(eq, order) - 1) / flux_chunk_of(eq, order)) *
+         flux_chunk_of(eq, order) * 64;
 }
 
 // Whether the volume phase keeps a copy of the tile's state in LDS (as it
@@ -187,12 +196,13 @@ __device__ constexpr int levi(int o, int x, int a) {
 // the RHS makes no round trip through HBM between the two (beta = 1 forward;
 // the reverse stage in the adjoint variables is res = q_hat, a = 1).
 template <int EQ, int ORDER, bool POST = false, bool STAGE = false>
-__global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
+__global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel(RhsArgs a) {
   using S = MfmaShape<ORDER>;
   using Flux = typename FluxOf<EQ>::type;
   constexpr int NP = S::NP, NFP = S::NFP, NFN = S::NFN, MB = S::MB, KS = S::KS;
   constexpr int NF = nfields_of(EQ);
-  constexpr int WAVES = kBlock / 64;
+  constexpr int WAVES = mfma_waves_of(EQ, ORDER);
+  constexpr int BLOCK = 64 * WAVES;
 
   constexpr bool SH = short_last_block(NP);   // last output block on 4x4x4
   constexpr int C = flux_chunk<EQ, ORDER>();
@@ -217,11 +227,11 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   double2* const fgl = s_fgeo[threadIdx.x >> 6];
 
   const int tid = threadIdx.x;
-  for (int k = tid; k < S::OP_TILES * 64; k += kBlock) s_op[k] = a.op_tiles[k];
-  for (int k = tid; k < S::LIFT_TILES * 64; k += kBlock) s_lift[k] = a.lift_tiles[k];
-  for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
+  for (int k = tid; k < S::OP_TILES * 64; k += BLOCK) s_op[k] = a.op_tiles[k];
+  for (int k = tid; k < S::LIFT_TILES * 64; k += BLOCK) s_lift[k] = a.lift_tiles[k];
+  for (int k = tid; k < NFN; k += BLOCK) s_fmask[k] = a.fmask[k];
   if constexpr (POST)
-    for (int k = tid; k < MB * KS * 64; k += kBlock) s_post[k] = a.post_tiles[k];
+    for (int k = tid; k < MB * KS * 64; k += BLOCK) s_post[k] = a.post_tiles[k];
   __syncthreads();
 
   const int lane = tid & 63;
@@ -243,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
   auto load_ext = [&](int64_t t, int ch) {
 #pragma unroll
     for (int it = 0; it < C; ++it)
-      ext_pf[it] = a.ext_t[t * flux_items_of(ORDER) + (ch * C + it) * 64 + lane];
+      ext_pf[it] = a.ext_t[t * flux_items_of(EQ, ORDER) + (ch * C + it) * 64 + lane];
   };
 
   for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
@@ -363,43 +373,48 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
           }
       }
     } else {
-      // E_t = curl H, H_t = -curl E, strong form: d_x F = sum_r rx[x][r] D_r F
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        f64x4 t[3][MB];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int b = 0; b < MB; ++b) t[r][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+      // E_t = curl H, H_t = -curl E in strong form.  rx is constant per
+      // element, so the Levi-Civita combination moves into the B operands:
+      //   (curl F)_o = sum_{x,a} eps_oxa d_x F_a = sum_r D_r w_{o,r},
+      //   w_{o,r} = sum_{x,a} eps_oxa rx[x][r] F_a,
+      // and every output field is one contraction per r, as in the
+      // divergence form (no per-field derivative accumulators).
 #pragma unroll KUNROLL
-        for (int ks = 0; ks < KS; ++ks) {
-          const int j = g * KS + ks;
-          const double u = j < NP ? a.q.p[c][el * NP + j] : 0.0;
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = g * KS + ks;
+        double f[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) f[c] = 0.0;
+        if (j < NP) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) f[c] = a.q.p[c][el * NP + j];
           if constexpr (ST)
-            if (j < NP) stg[c * 16 * NP + n * NP + j] = u;
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int b = 0; b < MB; ++b)
-              mma(t[r][b], s_op[((r * MB + b) * KS + ks) * 64 + lane], u, SH && b == MB - 1);
+            for (int c = 0; c < 6; ++c) stg[c * 16 * NP + n * NP + j] = f[c];
         }
-        const int comp = c % 3;
-        const bool is_h = c >= 3;
 #pragma unroll
-        for (int x = 0; x < 3; ++x) {
+        for (int r = 0; r < 3; ++r) {
+          double w[6];
 #pragma unroll
           for (int o = 0; o < 3; ++o) {
-            const int s = levi(o, x, comp);
-            if (s == 0) continue;
-            // E fields feed -curl E into H rhs; H fields feed curl H into E rhs
-            const double sign = is_h ? (double)s : -(double)s;
-            const int out = is_h ? o : 3 + o;
+            double ch = 0.0, ce = 0.0;
 #pragma unroll
-            for (int b = 0; b < MB; ++b)
+            for (int x = 0; x < 3; ++x)
 #pragma unroll
-              for (int v = 0; v < ncomp(SH && b == MB - 1); ++v)
-                acc[out][b][v] += sign * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
-                                          rx[3 * x + 2] * t[2][b][v]);
+              for (int c = 0; c < 3; ++c) {
+                const int s = levi(o, x, c);
+                if (s == 0) continue;
+                ch += (double)s * rx[3 * x + r] * f[3 + c];
+                ce += (double)s * rx[3 * x + r] * f[c];
+              }
+            w[o] = ch;         // E rhs: + curl H
+            w[3 + o] = -ce;    // H rhs: - curl E
+          }
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const double av = s_op[((r * MB + b) * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) mma(acc[c][b], av, w[c], SH && b == MB - 1);
           }
         }
       }
@@ -440,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mfma_kernel(RhsArgs a) {
             for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
           }
           const int32_t ext =
-              PF ? ext_pf[it] : a.ext_t[tile * flux_items_of(ORDER) + (ch * C + it) * 64 + lane];
+              PF ? ext_pf[it] : a.ext_t[tile * flux_items_of(EQ, ORDER) + (ch * C + it) * 64 + lane];
           if constexpr (branch_free_gathers<EQ, ORDER>()) {
             const int64_t src = ext >= 0 ? (int64_t)ext : vn;
 #pragma unroll

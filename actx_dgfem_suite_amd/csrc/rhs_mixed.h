@@ -419,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     // and the exterior map come with one contiguous load per slot; without,
     // all KSL k-steps are one chunk and the element-ordered tables are read.
     // BF16x3 pads the k-steps to KSL; the padding is zero.
-    constexpr int C = TT ? flux_chunk_of(ORDER) : KSL;
+    constexpr int C = TT ? flux_chunk_of(EQ, ORDER) : KSL;
     constexpr int NCH = TT ? NFP / C : 1;
     static_assert(!TT || NFP % C == 0, "whole flux chunks");
     if constexpr (TT) {
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
         for (int c = 0; c < NF; ++c)
           qm[c] = ST ? (double)stg[c * 16 * NP + fe * NP + s_fmask[fn]] : (double)a.q[c][vn];
-        const int32_t ext = TT ? a.ext_t[tile * flux_items_of(ORDER) + (ch * C + it) * 64 + lane]
+        const int32_t ext = TT ? a.ext_t[tile * flux_items_of(EQ, ORDER) + (ch * C + it) * 64 + lane]
                                : a.emap[pe * NFN + fn];
         if constexpr (mixed_branch_free_gathers<EQ, PREC, ORDER>()) {
           const int64_t src = ext >= 0 ? (int64_t)ext : vn;
