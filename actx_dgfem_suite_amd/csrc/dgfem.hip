@@ -546,10 +546,10 @@ std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp, int nface
 
 // A tiles of the mixed kernel (layout in rhs_mixed.h): 32-bit words, either
 // one float per lane (FP32) or four (hi, lo) bf16 pairs packed in 4 words.
-void build_mixed_tiles(int prec, int np, int nfp, const double* op, const double* lift,
+void build_mixed_tiles(int prec, int kgb, int np, int nfp, const double* op, const double* lift,
                        std::vector<uint32_t>& op_words, std::vector<uint32_t>& lift_words) {
-  const int kg = prec == DG_PREC_BF16X3 ? 4 : 1;
-  const int aw = prec == DG_PREC_BF16X3 ? 4 : 1;
+  const int kg = prec == DG_PREC_BF16X3 ? kgb : 1;
+  const int aw = kg;
   const int mb = (np + 15) / 16;
   const int ks_n = ((np + 3) / 4 + kg - 1) / kg * kg;
   const int ksl = (nfp + kg - 1) / kg * kg;
@@ -562,20 +562,21 @@ void build_mixed_tiles(int prec, int np, int nfp, const double* op, const double
       dst[base] = u;
       return;
     }
-    uint16_t hi[4], lo[4];
-    for (int i = 0; i < 4; ++i) {
+    // kg k per lane: words 0 .. kg/2-1 the hi parts, then the lo parts, two per word
+    uint16_t hi[8], lo[8];
+    for (int i = 0; i < kg; ++i) {
       const float f = (float)vals[i];
       hi[i] = bf16_bits(f);
       lo[i] = bf16_bits(f - bf16_value(hi[i]));
     }
-    dst[base + 0] = hi[0] | ((uint32_t)hi[1] << 16);
-    dst[base + 1] = hi[2] | ((uint32_t)hi[3] << 16);
-    dst[base + 2] = lo[0] | ((uint32_t)lo[1] << 16);
-    dst[base + 3] = lo[2] | ((uint32_t)lo[3] << 16);
+    for (int w = 0; w < kg / 2; ++w) {
+      dst[base + w] = hi[2 * w] | ((uint32_t)hi[2 * w + 1] << 16);
+      dst[base + kg / 2 + w] = lo[2 * w] | ((uint32_t)lo[2 * w + 1] << 16);
+    }
   };
   op_words.assign((size_t)3 * mb * (ks_n / kg) * 64 * aw, 0u);
   lift_words.assign((size_t)mb * (ksl / kg) * 64 * aw, 0u);
-  double vals[4];
+  double vals[8];
   for (int r = 0; r < 3; ++r)
     for (int b = 0; b < mb; ++b)
       for (int kq = 0; kq < ks_n / kg; ++kq)
@@ -1224,7 +1225,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   for (int k = 0; k < 2 && !tri; ++k) {
     const int prec = k == 0 ? DG_PREC_FP32 : DG_PREC_BF16X3;
     std::vector<uint32_t> ow, lw;
-    build_mixed_tiles(prec, np, nfp, dsrc, d->lift, ow, lw);
+    build_mixed_tiles(prec, bf16_kgroup_of(d->equation, d->order), np, nfp, dsrc, d->lift, ow, lw);
     if ((rc = upload(&p->mixed_op[k], ow.data(), ow.size())) ||
         (rc = upload(&p->mixed_lift[k], lw.data(), lw.size()))) {
       free_plan(p);

@@ -1,7 +1,8 @@
 // Mixed-precision fused RHS kernel: FP32 state in and RHS out, the D and LIFT
 // contractions on the matrix cores in FP32 (v_mfma_f32_16x16x4f32) or as a
-// three-product bf16 split (v_mfma_f32_16x16x16bf16_1k) with FP32
-// accumulation, and everything else -- geometric factors, the numerical flux
+// three-product bf16 split (gfx950's v_mfma_f32_16x16x32_bf16, or
+// v_mfma_f32_16x16x16bf16_1k where the wider form's padding costs occupancy)
+// with FP32 accumulation, and everything else -- geometric factors, the numerical flux
 // (Euler pressure recovery included) and the combination of reference
 // derivatives into physical ones -- in FP64 on values promoted from FP32.
 //
@@ -12,12 +13,14 @@
 // buffer as contiguous stores.
 //
 // Operand layouts (lane l, g = l / 16, n = l % 16), measured on gfx950:
-//   f32 16x16x4:       A[m = n][k = g], B[k = g][col = n], C[row = 4 g + v][col = n]
-//   bf16 16x16x16_1k:  lane l supplies 4 consecutive k = 4 g + i (i = 0..3) of
-//                      A[m = n][k] and B[k][col = n]; C as for f32.
+//   f32 16x16x4:    A[m = n][k = g], B[k = g][col = n], C[row = 4 g + v][col = n]
+//   bf16 16x16x32:  lane l supplies 8 consecutive k = 8 g + i (i = 0..7) of
+//                   A[m = n][k] and B[k][col = n]; C as for f32.
 // Contraction index k at FP32 k-step ks stands for node j = g * KS + ks (a
 // contiguous run per lane group, as in the FP64 kernel); the bf16 form takes
-// four consecutive k-steps per instruction (KS rounded up to a multiple of 4).
+// eight consecutive k-steps per instruction (KS rounded up to a multiple of
+// 8).  On MI355X the 16x16x32 form runs 1.94 PFLOP/s dense against 1.06 for
+// CDNA3's 16x16x16_1k (same time per instruction, twice the k).
 //
 // BF16x3: x = x_hi + x_lo with x_hi = bf16(x), x_lo = bf16(x - x_hi); the
 // product is a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, ~2^-16
@@ -30,7 +33,11 @@
 namespace dg {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 struct MixedArgs {
   Schedule sched;           // 16-element tiles
@@ -72,12 +79,31 @@ __host__ __device__ inline float bf16_value(uint16_t b) {
   return x;
 }
 
-template <int PREC>
+// bf16 k-steps per instruction of the BF16x3 form: 8 on gfx950's
+// v_mfma_f32_16x16x32_bf16 or 4 on v_mfma_f32_16x16x16bf16_1k.  The 16x16x32
+// form runs twice the flops per instruction (1.94 against 1.06 PFLOP/s
+// measured on MI355X), but pads k to 8 per lane group in the A tiles, the
+// flux buffer and the on-the-fly B split, and the contractions are not what
+// bounds these kernels: measured, it is 1.5% faster at Maxwell P4 and even to
+// 45% slower elsewhere (wave P2 +22%, Euler P1 +43%, Maxwell P1 +8%).
+__host__ __device__ constexpr int bf16_kgroup_of(int eq, int order) {
+  return (eq == DG_EQ_MAXWELL && order == 4) ? 8 : 4;
+}
+
+// Whether the B split uses gfx950's v_cvt_pk_bf16_f32 (two values per
+// instruction) rather than integer rounding (bf16_bits, about 11 VALU
+// operations per value).  Measured on MI355X: Euler P1 -8%, P2 -5%, P4 -5%,
+// Maxwell P1 -6%, wave even; at Maxwell P2 it costs a wave per SIMD (+37%).
+__host__ __device__ constexpr bool bf16_hw_split_of(int eq, int order) {
+  return !(eq == DG_EQ_MAXWELL && order == 2);
+}
+
+template <int PREC, int KGB, bool HW = true>
 struct MixedMma;
 
 // FP32: one k per lane per instruction
-template <>
-struct MixedMma<DG_PREC_FP32> {
+template <int KGB, bool HW>
+struct MixedMma<DG_PREC_FP32, KGB, HW> {
   static constexpr int KG = 1;                  // k-steps per instruction
   static constexpr int A_WORDS = 1;             // 32-bit words of A per lane
   __device__ static int row(int g, int v) { return 4 * g + v; }
@@ -86,11 +112,67 @@ struct MixedMma<DG_PREC_FP32> {
   }
 };
 
-// BF16x3: four k per lane per instruction, A as (hi[4], lo[4]) bf16
+// BF16x3 split of the B operand, KG values -> packed hi and lo words: the
+// conversions are gfx950's v_cvt_pk_bf16_f32 (round to nearest even, as the
+// host's split of A), two values per instruction
+template <int KG>
+__device__ inline void split_bf16(const float* b, uint32_t* hw, uint32_t* lw) {
+#pragma unroll
+  for (int i = 0; i < KG / 2; ++i) {
+    const bf16x2 h = {(__bf16)b[2 * i], (__bf16)b[2 * i + 1]};
+    const bf16x2 l = {(__bf16)(b[2 * i] - (float)h[0]), (__bf16)(b[2 * i + 1] - (float)h[1])};
+    hw[i] = __builtin_bit_cast(uint32_t, h);
+    lw[i] = __builtin_bit_cast(uint32_t, l);
+  }
+}
+
+// BF16x3 on v_mfma_f32_16x16x32_bf16: eight k per lane per instruction, A as
+// (hi[8], lo[8]) bf16 in 8 words
+template <bool HW>
+struct MixedMma<DG_PREC_BF16X3, 8, HW> {
+  static constexpr int KG = 8;
+  static constexpr int A_WORDS = 8;             // hi: 4 words, lo: 4 words
+  __device__ static int row(int g, int v) { return 4 * g + v; }
+  __device__ static f32x4 mma(const uint32_t* a, const float* b, f32x4 c) {
+    uint32_t hw[4], lw[4];
+    static_assert(HW, "the 16x16x32 form splits with v_cvt_pk_bf16_f32");
+    split_bf16<8>(b, hw, lw);
+    const bf16x8 ahi = __builtin_bit_cast(bf16x8, (u32x4){a[0], a[1], a[2], a[3]});
+    const bf16x8 alo = __builtin_bit_cast(bf16x8, (u32x4){a[4], a[5], a[6], a[7]});
+    const bf16x8 bhi = __builtin_bit_cast(bf16x8, (u32x4){hw[0], hw[1], hw[2], hw[3]});
+    const bf16x8 blo = __builtin_bit_cast(bf16x8, (u32x4){lw[0], lw[1], lw[2], lw[3]});
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, c, 0, 0, 0);
+  }
+};
+
+// BF16x3 on v_mfma_f32_16x16x16bf16_1k: four k per lane per instruction, A as
+// (hi[4], lo[4]) bf16 in 4 words; B split by v_cvt_pk_bf16_f32
 template <>
-struct MixedMma<DG_PREC_BF16X3> {
+struct MixedMma<DG_PREC_BF16X3, 4, true> {
   static constexpr int KG = 4;
   static constexpr int A_WORDS = 4;             // hi: 2 words, lo: 2 words
+  __device__ static int row(int g, int v) { return 4 * g + v; }
+  __device__ static f32x4 mma(const uint32_t* a, const float* b, f32x4 c) {
+    uint32_t hw[2], lw[2];
+    split_bf16<4>(b, hw, lw);
+    const bf16x4 ahi = __builtin_bit_cast(bf16x4, (u32x2){a[0], a[1]});
+    const bf16x4 alo = __builtin_bit_cast(bf16x4, (u32x2){a[2], a[3]});
+    const bf16x4 bhi = __builtin_bit_cast(bf16x4, (u32x2){hw[0], hw[1]});
+    const bf16x4 blo = __builtin_bit_cast(bf16x4, (u32x2){lw[0], lw[1]});
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, c, 0, 0, 0);
+  }
+};
+
+// The same with the B split by integer rounding, operand by operand (the
+// form that keeps Maxwell P2 at two waves per SIMD)
+template <>
+struct MixedMma<DG_PREC_BF16X3, 4, false> {
+  static constexpr int KG = 4;
+  static constexpr int A_WORDS = 4;
   __device__ static int row(int g, int v) { return 4 * g + v; }
   __device__ static f32x4 mma(const uint32_t* a, const float* b, f32x4 c) {
     bf16x4 ahi, alo, bhi, blo;
@@ -110,16 +192,16 @@ struct MixedMma<DG_PREC_BF16X3> {
   }
 };
 
-template <int PREC, int ORDER>
+template <int PREC, int ORDER, int KGB>
 struct MixedShape {
   static constexpr int NP = np_of(ORDER);
   static constexpr int NFP = nfp_of(ORDER);
   static constexpr int NFN = kFaces * NFP;
   static constexpr int MB = (NP + 15) / 16;
-  static constexpr int KG = MixedMma<PREC>::KG;
+  static constexpr int KG = MixedMma<PREC, KGB>::KG;
   static constexpr int KS = ((NP + 3) / 4 + KG - 1) / KG * KG;  // node k-steps per lane group
   static constexpr int KSL = (NFP + KG - 1) / KG * KG;         // LIFT k-steps per face
-  static constexpr int AW = MixedMma<PREC>::A_WORDS;
+  static constexpr int AW = MixedMma<PREC, KGB>::A_WORDS;
   // A tiles, 32-bit words: [r][b][ks / KG][lane][AW] and [b][ks / KG][lane][AW]
   static constexpr int OP_WORDS = 3 * MB * (KS / KG) * 64 * AW;
   static constexpr int LIFT_WORDS = MB * (KSL / KG) * 64 * AW;
@@ -191,8 +273,9 @@ constexpr bool mixed_stage_preload() {
 // phase (the FP32 RHS tile is the one the unfused path would store).
 template <int EQ, int PREC, int ORDER, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
-  using S = MixedShape<PREC, ORDER>;
-  using M = MixedMma<PREC>;
+  constexpr int KGB = bf16_kgroup_of(EQ, ORDER);
+  using S = MixedShape<PREC, ORDER, KGB>;
+  using M = MixedMma<PREC, KGB, bf16_hw_split_of(EQ, ORDER)>;
   using Flux = typename FluxOf<EQ>::type;
   constexpr int NP = S::NP, NFP = S::NFP, NFN = S::NFN, MB = S::MB;
   constexpr int KS = S::KS, KSL = S::KSL, KG = S::KG, AW = S::AW;
