@@ -268,12 +268,24 @@ constexpr bool mixed_stage_preload() {
   return !(EQ == DG_EQ_MAXWELL && ORDER == 2);
 }
 
+// Whether the Maxwell volume phase contracts the curl's combination
+// w_{o,r} = sum_{x,a} eps_oxa rx[x][r] F_a (as the FP64 kernel does) rather
+// than each field's reference derivatives, recombined after.  Measured on
+// MI355X: FP32 P1 -12%, P2 -19%, P3 -11%, P4 -29%; BF16x3 P3 -6%, but P1
+// +10% and P4 +6%, where splitting 18 combined B operands per node instead
+// of 6 state values costs more than the recombination it saves.
+template <int PREC, int ORDER>
+constexpr bool mixed_maxwell_bform() {
+  return PREC == DG_PREC_FP32 || ORDER == 3;
+}
+
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
 // res = a res + dt rhs, q = q + b res, q32 = (float)q to it in its output
 // phase (the FP32 RHS tile is the one the unfused path would store).
 template <int EQ, int PREC, int ORDER, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   constexpr int KGB = bf16_kgroup_of(EQ, ORDER);
+  constexpr bool MXB = mixed_maxwell_bform<PREC, ORDER>();
   using S = MixedShape<PREC, ORDER, KGB>;
   using M = MixedMma<PREC, KGB, bf16_hw_split_of(EQ, ORDER)>;
   using Flux = typename FluxOf<EQ>::type;
@@ -424,6 +436,55 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
             for (int c = 0; c < 5; ++c) acc[c][b] = M::mma(aw, wb[c][r], acc[c][b]);
           }
+      } else if constexpr (MXB) {
+        // Maxwell: E_t = curl H, H_t = -curl E with the curl's rx /
+        // Levi-Civita combination in B-operand space, as in the FP64 kernel:
+        // (curl F)_o = sum_r D_r w_{o,r}, w_{o,r} = sum_{x,a} eps_oxa rx[x][r] F_a
+        double f[6][KG];
+#pragma unroll
+        for (int i = 0; i < KG; ++i) {
+          const int j = g * KS + kq * KG + i;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) f[c][i] = 0.0;
+          if (j < NP) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+              const float v = a.q[c][el * NP + j];
+              f[c][i] = v;
+              if constexpr (ST) stg[c * 16 * NP + n * NP + j] = v;
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          float wb[6][KG];
+#pragma unroll
+          for (int i = 0; i < KG; ++i)
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+              double ch = 0.0, ce = 0.0;
+#pragma unroll
+              for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  const int sg = levi(o, x, c);
+                  if (sg == 0) continue;
+                  ch += (double)sg * rx[3 * x + r] * f[3 + c][i];
+                  ce += (double)sg * rx[3 * x + r] * f[c][i];
+                }
+              wb[o][i] = (float)ch;
+              wb[3 + o][i] = (float)(-ce);
+            }
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const uint32_t* av = a_op(r, b, kq);
+            uint32_t aw[AW];
+#pragma unroll
+            for (int w = 0; w < AW; ++w) aw[w] = av[w];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c][b] = M::mma(aw, wb[c], acc[c][b]);
+          }
+        }
       }
     }
     if constexpr (EQ == DG_EQ_WAVE) {
@@ -438,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
             acc[1 + x][b][v] = (float)(-c0 * (rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2));
         }
     }
-    if constexpr (EQ == DG_EQ_MAXWELL) {
+    if constexpr (EQ == DG_EQ_MAXWELL && !MXB) {
       // E_t = curl H, H_t = -curl E from t_{c,r} = D_r F_c, one field at a time
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
