@@ -57,9 +57,14 @@ class DGPlan:
         self.device = torch.device("cuda", torch.cuda.current_device()
                                    if device is None else torch.device(device).index or 0)
         ref = discr.ref
-        centroids = discr.nodes.mean(axis=2).T
-        if centroids.shape[1] == 2:
-            centroids = np.concatenate([centroids, np.zeros((centroids.shape[0], 1))], axis=1)
+        # element centroids: the plan's locality schedule (a discretization
+        # without node coordinates keeps its element order)
+        centroids = None
+        if getattr(discr, "nodes", None) is not None:
+            centroids = discr.nodes.mean(axis=2).T
+            if centroids.shape[1] == 2:
+                centroids = np.concatenate([centroids, np.zeros((centroids.shape[0], 1))],
+                                           axis=1)
         self._host = {
             "d_weak": np.ascontiguousarray(ref.d_weak, dtype=np.float64),
             "d_strong": np.ascontiguousarray(ref.d_strong, dtype=np.float64),
@@ -69,8 +74,8 @@ class DGPlan:
             "ext_map": discr.packed_ext_map(),
             "mass_inv": np.ascontiguousarray(ref.mass_inv, dtype=np.float64),
             "face_mass": np.ascontiguousarray(ref.face_mass, dtype=np.float64),
-            # element centroids: the plan's locality schedule
-            "centroids": np.ascontiguousarray(centroids, dtype=np.float64),
+            "centroids": (None if centroids is None
+                          else np.ascontiguousarray(centroids, dtype=np.float64)),
         }
         h = self._host
         desc = _lib.DgPlanDesc()
@@ -86,7 +91,8 @@ class DGPlan:
         desc.mass_inv = _dptr(h["mass_inv"], ctypes.c_double)
         desc.face_mass = _dptr(h["face_mass"], ctypes.c_double)
         desc.dim = self.dim
-        desc.centroids = _dptr(h["centroids"], ctypes.c_double)
+        if h["centroids"] is not None:
+            desc.centroids = _dptr(h["centroids"], ctypes.c_double)
         desc.params[0] = self.param
         desc.variant = _lib.KERNEL_VARIANTS[variant]
         handle = ctypes.c_void_p()

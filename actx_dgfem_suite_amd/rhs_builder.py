@@ -25,8 +25,16 @@ def parse_equation(equation: str) -> tuple[int, str]:
 
 
 def get_rhs(equation: str, actx, dim: int, degree: int) -> tuple[Any, tuple[Any, ...]]:
-    """Return ``(rhs_callable, call_args)`` for ``equation``."""
+    """Return ``(rhs_callable, call_args)`` for ``equation``.  On the tracing
+    context the callable is the operator written against the grudge-shaped
+    layer (``lazy.operators``), which the context traces and matches."""
     ndofs, base = parse_equation(equation)
+    from actx_dgfem_suite_amd.lazy import HIPDGFEMLazyArrayContext
+    if isinstance(actx, HIPDGFEMLazyArrayContext):
+        from actx_dgfem_suite_amd.lazy import operators as lazy_ops
+        builder = {"wave": lazy_ops.get_wave_rhs, "euler": lazy_ops.get_euler_rhs,
+                   "maxwell": lazy_ops.get_maxwell_rhs}[base]
+        return builder(actx=actx, dim=dim, order=degree, ndofs=ndofs)
     if base == "wave":
         from actx_dgfem_suite_amd.equations.wave import get_wave_rhs
         return get_wave_rhs(actx=actx, dim=dim, order=degree, ndofs=ndofs)

@@ -37,6 +37,7 @@ from actx_dgfem_suite_amd.arraycontext import (
     HIPDGFEMLDSArrayContext,
 )
 from actx_dgfem_suite_amd.containers import flatten_fields
+from actx_dgfem_suite_amd.lazy import HIPDGFEMLazyArrayContext
 from actx_dgfem_suite_amd.measure import finish_command_queue, instantiate_actx_t
 from actx_dgfem_suite_amd.perf_analysis import get_float64_flops, get_roofline_flop_rate
 from actx_dgfem_suite_amd.rhs_builder import get_rhs
@@ -47,6 +48,8 @@ _NAME_TO_ACTX_CLASS = {
     "hip:dgfem_lds": HIPDGFEMLDSArrayContext,
     "hip:dgfem_fp32": HIPDGFEMFP32ArrayContext,
     "hip:dgfem_bf16x3": HIPDGFEMBF16x3ArrayContext,
+    # traces the operator into an einsum DAG, normalises and matches it
+    "hip:dgfem_lazy": HIPDGFEMLazyArrayContext,
 }
 _ACTX_CLASS_TO_NAME = {v: k for k, v in _NAME_TO_ACTX_CLASS.items()}
 

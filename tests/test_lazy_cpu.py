@@ -1,0 +1,219 @@
+"""The tracing path on the CPU: operators written against the grudge-shaped
+layer, traced into a DAG, normalised and matched.
+
+* The traced (unfused) DAG, evaluated on the host by the test's own
+  evaluator, equals the oracle's RHS: the grudge-shaped operators are the
+  reference's operators.
+* The normalised DAG (inverse mass distributed and fused, constants folded)
+  evaluates to the same values.
+* The matcher recovers the equation, its parameter and every array the plan
+  takes (operators, geometry, face maps) from the DAG alone.
+* Operators the kernels do not implement are refused with
+  ``NotImplementedError``: a central or one-sided flux, a wrong wall state, an
+  extra nonlinear term, an inverse mass of something else.
+* An equivalent operator spelled differently (boundary state by selection
+  instead of face-set projections) still matches.
+"""
+
+import numpy as np
+import pytest
+
+from actx_dgfem_suite_amd.containers import ConservedEulerField, WaveState, flatten_fields, obj_array
+from actx_dgfem_suite_amd.lazy import HIPDGFEMLazyArrayContext, Unmatched, ir, match_dg_operator, normalize
+from actx_dgfem_suite_amd.lazy import grudge_ops as op
+from actx_dgfem_suite_amd.lazy import operators as lo
+from actx_dgfem_suite_amd.lazy.grudge_ops import LazyDiscretizationCollection, TracePair
+from tests._helpers import discr_pair, oracle_rhs, random_state
+from tests._lazy_eval import evaluate
+
+NF = {"wave": 4, "euler": 5, "maxwell": 6}
+
+
+def _trace(equation, order, param, operator=None, nel=2):
+    d, od = discr_pair(order, nel)
+    actx = HIPDGFEMLazyArrayContext()
+    dcoll = LazyDiscretizationCollection(actx, d)
+    phs = [ir.Placeholder(f"q{k}", (d.nelements, d.np)) for k in range(NF[equation])]
+    if equation == "wave":
+        state = WaveState(u=phs[0], v=obj_array(phs[1:]))
+        out = (operator or lo.wave_operator)(actx, dcoll, param, state)
+    elif equation == "euler":
+        state = ConservedEulerField(mass=phs[0], energy=phs[1], momentum=obj_array(phs[2:]))
+        out = (operator or lo.euler_operator)(actx, dcoll, param, state)
+    else:
+        out = (operator or lo.maxwell_operator)(actx, dcoll, param, obj_array(phs))
+    return d, od, actx, dcoll, phs, flatten_fields(out)
+
+
+@pytest.mark.parametrize("equation,param", [("wave", 1.3), ("euler", 1.4), ("maxwell", 0.5)])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_traced_operator_and_normalisation_match_the_oracle(equation, param, order):
+    d, od, _, _, phs, outs = _trace(equation, order, param)
+    q = random_state(equation, od.x, seed=order)
+    values = {p.name: q[k] for k, p in enumerate(phs)}
+    ref = oracle_rhs(equation, od, q, param)
+    got = np.stack(evaluate(outs, values))
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-12 * scale
+    norm = normalize(outs)
+    assert norm.unfused == 0
+    fused = np.stack(evaluate(norm.outputs, values))
+    assert np.abs(fused - got).max() < 1e-12 * scale
+    # after fusion no inverse-mass einsum is left
+    for node in ir.walk(norm.outputs):
+        assert not (isinstance(node, ir.Einsum) and node.spec == "e,ij,ej->ei")
+
+
+@pytest.mark.parametrize("equation,param", [("wave", 1.0), ("wave", 2.5), ("euler", 1.4),
+                                            ("euler", 1.67), ("maxwell", 0.5), ("maxwell", 1.0),
+                                            ("maxwell", 0.3)])
+@pytest.mark.parametrize("order", [1, 3])
+def test_matcher_recovers_equation_parameter_and_plan_arrays(equation, param, order):
+    d, _, _, _, phs, outs = _trace(equation, order, param)
+    m = match_dg_operator(normalize(outs), phs)
+    assert m.equation == equation
+    assert abs(m.param - param) < 1e-12 * param
+    assert m.order == order
+    ref = d.ref
+    for got, want in [(m.d_weak, ref.d_weak), (m.d_strong, ref.d_strong), (m.lift, ref.lift),
+                      (m.mass_inv, ref.mass_inv), (m.face_mass, ref.face_mass),
+                      (m.geometry, d.packed_geometry())]:
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    assert np.array_equal(m.face_nodes, ref.face_nodes)
+    assert np.array_equal(m.ext_map, d.packed_ext_map())
+
+
+def _wave_where_form(actx, dcoll, c, w):
+    """The upwind wave operator with the boundary state chosen by selection
+    on every face and the flux written through the traces' difference."""
+    u, v = w.u, list(w.v)
+    n = op.normal(dcoll)
+    b = dcoll.bdry_mask
+    ui, ue = ir.Take(u, dcoll.index_int), ir.Take(u, dcoll.index_ext)
+    vi = [ir.Take(x, dcoll.index_int) for x in v]
+    ve = [ir.Take(x, dcoll.index_ext) for x in v]
+    ue = actx.np.where(b, -ui, ue)
+    ve = [actx.np.where(b, a, e) for a, e in zip(vi, ve)]
+    vn_sum = sum((vi[x] + ve[x]) * n[x] for x in range(3))
+    vn_jump = sum((ve[x] - vi[x]) * n[x] for x in range(3))
+    flux_u = 0.5 * c * vn_sum + 0.5 * c * (ue - ui)
+    flux_v = [c * 0.5 * (ui + ue) * n[x] + c * 0.5 * vn_jump * n[x] for x in range(3)]
+    rhs_u = op.inverse_mass(dcoll, op.face_mass(dcoll, flux_u) - c * op.weak_local_div(dcoll, v))
+    grads = op.weak_local_grad(dcoll, u)
+    rhs_v = [op.inverse_mass(dcoll, op.face_mass(dcoll, flux_v[x]) - c * grads[x])
+             for x in range(3)]
+    return WaveState(u=rhs_u, v=obj_array(rhs_v))
+
+
+def test_equivalent_spelling_matches():
+    d, od, _, _, phs, outs = _trace("wave", 2, 1.7, operator=_wave_where_form)
+    q = random_state("wave", od.x, seed=3)
+    got = np.stack(evaluate(outs, {p.name: q[k] for k, p in enumerate(phs)}))
+    ref = oracle_rhs("wave", od, q, 1.7)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    m = match_dg_operator(normalize(outs), phs)
+    assert m.equation == "wave" and abs(m.param - 1.7) < 1e-12
+
+
+def _central_wave(actx, dcoll, c, w):
+    """No upwind terms: the kernels do not implement it."""
+    u, v = w.u, list(w.v)
+    n = op.normal(dcoll)
+    pu = op.interior_trace_pair(dcoll, u)
+    pv = [op.interior_trace_pair(dcoll, x) for x in v]
+    du = op.boundary_trace(dcoll, u)
+    bd = [op.project_boundary_to_all_faces(dcoll, -du * 0.0)]
+    flux_u = op.project_interior_to_all_faces(dcoll, sum(pv[x].avg * n[x] for x in range(3)))
+    flux_v = [op.project_interior_to_all_faces(dcoll, pu.avg * n[x]) for x in range(3)]
+    rhs_u = op.inverse_mass(dcoll, -c * op.weak_local_div(dcoll, v)
+                            + op.face_mass(dcoll, flux_u + bd[0]))
+    grads = op.weak_local_grad(dcoll, u)
+    rhs_v = [op.inverse_mass(dcoll, -c * grads[x] + op.face_mass(dcoll, flux_v[x]))
+             for x in range(3)]
+    return WaveState(u=rhs_u, v=obj_array(rhs_v))
+
+
+def _one_sided_euler(actx, dcoll, gamma, q):
+    """LF with the wavespeed of the interior trace only."""
+    def one_sided_flux(actx_, dcoll_, qi, qe, g):
+        n = op.normal(dcoll_)
+        fi = lo.euler_flux_tensor(actx_, qi, g)
+        fe = lo.euler_flux_tensor(actx_, qe, g)
+        lam = lo._wavespeed(actx_, qi, g)
+        return [lo._dot([0.5 * (fi[c][x] + fe[c][x]) for x in range(3)], n)
+                + 0.5 * lam * (qi[c] - qe[c]) for c in range(5)]
+
+    orig = lo.euler_lf_flux
+    lo.euler_lf_flux = one_sided_flux
+    try:
+        return lo.euler_operator(actx, dcoll, gamma, q)
+    finally:
+        lo.euler_lf_flux = orig
+
+
+def _wave_with_nonlinear_volume(actx, dcoll, c, w):
+    out = lo.wave_operator(actx, dcoll, c, w)
+    extra = op.inverse_mass(dcoll, op.weak_local_div(dcoll, [w.u * w.u, w.u * 0.0, w.u * 0.0]))
+    return WaveState(u=out.u + extra, v=out.v)
+
+
+def _wave_with_stray_inverse_mass(actx, dcoll, c, w):
+    out = lo.wave_operator(actx, dcoll, c, w)
+    return WaveState(u=out.u + op.inverse_mass(dcoll, w.u), v=out.v)
+
+
+def _maxwell_wrong_wall(actx, dcoll, alpha, w):
+    """E+ = E- on the walls instead of PEC."""
+    fields = flatten_fields(w)
+    bt = op.boundary_trace
+    grads = [op.local_grad(dcoll, f) for f in fields]
+
+    def curl(g):
+        return [g[2][1] - g[1][2], g[0][2] - g[2][0], g[1][0] - g[0][1]]
+
+    eb = [bt(dcoll, f) for f in fields[0:3]]
+    hb = [bt(dcoll, f) for f in fields[3:6]]
+    bdry = lo.maxwell_flux(dcoll, alpha, [TracePair(int=x, ext=x) for x in eb],
+                           [TracePair(int=x, ext=x) for x in hb])
+    pairs = [op.interior_trace_pair(dcoll, f) for f in fields]
+    inner = lo.maxwell_flux(dcoll, alpha, pairs[0:3], pairs[3:6])
+    flux = lo._faces(dcoll, bdry, inner)
+    strong = curl(grads[3:6]) + [-x for x in curl(grads[0:3])]
+    return obj_array([strong[c] - op.inverse_mass(dcoll, op.face_mass(dcoll, flux[c]))
+                      for c in range(6)])
+
+
+@pytest.mark.parametrize("equation,operator", [
+    ("wave", _central_wave),
+    ("euler", _one_sided_euler),
+    ("wave", _wave_with_nonlinear_volume),
+    ("wave", _wave_with_stray_inverse_mass),
+    ("maxwell", _maxwell_wrong_wall),
+])
+def test_operators_the_kernels_do_not_implement_are_refused(equation, operator):
+    _, _, _, _, phs, outs = _trace(equation, 2, {"wave": 1.0, "euler": 1.4,
+                                                 "maxwell": 0.5}[equation], operator=operator)
+    with pytest.raises(NotImplementedError, match="unmatched DG operator"):
+        match_dg_operator(normalize(outs), phs)
+    assert issubclass(Unmatched, NotImplementedError)
+
+
+def test_compile_refuses_an_unmatched_callable_before_any_device_work():
+    """The trace and the match run on the host; an unmatched DAG raises
+    before a plan is made."""
+    d, _ = discr_pair(1, 2)
+    actx = HIPDGFEMLazyArrayContext()
+    dcoll = LazyDiscretizationCollection(actx, d)
+
+    def rhs(w):
+        return _central_wave(actx, dcoll, 1.0, w)
+
+    compiled = actx.compile(rhs)
+    torch = pytest.importorskip("torch")
+    state = WaveState(u=torch.zeros(d.nelements, d.np, dtype=torch.float64),
+                      v=obj_array([torch.zeros(d.nelements, d.np, dtype=torch.float64)
+                                   for _ in range(3)]))
+    with pytest.raises(NotImplementedError):
+        compiled(state)
+    assert compiled.plan is None

@@ -101,7 +101,7 @@ def test_runner_command_line():
                 "--actxs", "hip:dgfem,hip:dgfem_lds", "--no-verify"])
     assert a.no_verify and a.equations == "wave,tiny_euler"
     assert set(_NAME_TO_ACTX_CLASS) == {"hip:dgfem", "hip:dgfem_lds", "hip:dgfem_fp32",
-                                         "hip:dgfem_bf16x3"}
+                                         "hip:dgfem_bf16x3", "hip:dgfem_lazy"}
 
 
 def test_dof_scaling_command_line():
