@@ -6,10 +6,10 @@ flux; 1000 timed RHS evaluations.  A "step" is one RHS evaluation
 ``rhs = L(q)`` with ``q`` already resident in HBM.
 
 Prints one JSON line with the metric, the roofline position of the fused RHS
-kernel and a CPU baseline (the numpy oracle on a bounded sample of the same
-workload).  The kernel time is one HIP event pair on the launch stream
-around all K back-to-back launches, divided by K: it includes the gaps
-between consecutive launches.
+kernel and a CPU baseline (the oracle's OpenMP C restatement of the fused
+RHS on the same workload, all host cores up to 16).  The kernel time is one
+HIP event pair on the launch stream around all K back-to-back launches,
+divided by K: it includes the gaps between consecutive launches.
 
     python bench.py [--steps 1000] [--warmup 20] [--workload wave_p3]
 """
@@ -21,6 +21,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -71,50 +73,32 @@ def measured_traffic(workload: str, kernel: str) -> int | None:
     return None
 
 
-def cpu_baseline(workload, seconds: float):
-    """Time the numpy oracle on a bounded sample: the same equation and order
-    on a smaller box, reported as evaluations/s of the full workload (scaled
-    by the element count)."""
-    import numpy as np
-    from threadpoolctl import threadpool_limits
-
-    from actx_dgfem_suite_amd.discretization import make_box_discretization
-    from actx_dgfem_suite_amd.workloads import initial_state
-    from oracle import dg_oracle
+def cpu_baseline(workload, d, q, seconds: float):
+    """Time the oracle's OpenMP C restatement of the fused RHS (oracle/dg_cpu.c,
+    checked against the numpy oracle in tests/) on the same workload, with one
+    thread per host core up to 16 (the box's CPU share), for about ``seconds``
+    of wall time."""
+    from oracle import dg_cpu
 
     threads = min(os.cpu_count() or 1, 16)
-    nel = max(2, min(workload.nel_1d, 10))
-    d = make_box_discretization(workload.order, nel)
-    od = dg_oracle.build_discretization(d.mesh.vertices, d.mesh.elements, d.ref.nodes)
-    q = initial_state(workload, d)
-
-    def one():
-        if workload.equation == "wave":
-            dg_oracle.wave_rhs(od, q[0], q[1:4], c=workload.param)
-        elif workload.equation == "euler":
-            dg_oracle.euler_rhs(od, q, gamma=workload.param)
-        else:
-            dg_oracle.maxwell_rhs(od, q, alpha=workload.param)
-
-    with threadpool_limits(threads):
-        one()
-        n, t0 = 0, time.perf_counter()
-        while True:
-            one()
-            n += 1
-            el = time.perf_counter() - t0
-            if el >= seconds or n >= 10000:
-                break
-    full_k = 6 * workload.nel_1d ** 3
-    per_eval_full = el / n * full_k / d.nelements
+    op = dg_cpu.CpuOperator(d, workload.equation, workload.param, threads=threads)
+    out = np.empty_like(q)
+    op(q, out)                                   # warm-up (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        op(q, out)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 1000:
+            break
     return {
-        "value": 1.0 / per_eval_full,
+        "value": n / el,
         "unit": "RHS evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"numpy oracle, {workload.equation} P{workload.order} on "
-                   f"{nel}^3x6={d.nelements} tets, {n} evals in {el:.1f} s, "
-                   f"scaled by elements to {full_k} tets"),
+        "sample": (f"OpenMP C restatement of the fused RHS (oracle/dg_cpu.c), "
+                   f"{workload.equation} P{workload.order} on the full {d.nelements}-tet "
+                   f"mesh, {n} evals in {el:.1f} s, {threads} threads"),
     }
 
 
@@ -333,8 +317,8 @@ def main():
     if adjoint is not None:
         result["adjoint"] = adjoint
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline sample ...", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(w, d, initial_state(w, d), args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 
