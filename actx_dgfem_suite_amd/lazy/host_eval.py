@@ -1,5 +1,14 @@
-"""Host evaluation of a lazy DAG (test infrastructure: the checker of the
-traced operators and of the normalisation, never a product path)."""
+"""Host evaluation of a lazy DAG with numpy.
+
+It serves as a checker only:
+- the runner's value check evaluates the traced operator with it, the
+  counterpart of the reference's check against its ``NumpyArrayContext``
+  (run/__main__.py:77-96);
+- the tests check the traced operators and the normalisation with it.
+
+No compiled callable evaluates through it: ``HIPDGFEMLazyArrayContext``
+runs the matched HIP plan and nothing else, and fails without a GPU.
+"""
 
 from __future__ import annotations
 
