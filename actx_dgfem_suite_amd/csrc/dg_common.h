@@ -52,6 +52,7 @@ struct RhsArgs {
   const double* vgeo;        // [tile][9][16]: rx of element n of the tile
   const double* fgeo;        // [tile][face*16 + n][4]: n_x, n_y, n_z, Fscale
   const int32_t* ext_t;      // [tile][chunk][item][64]: emap in flux-item order
+  const int32_t* ext_d;      // [tile][kstep][64]: emap in the direct flux order
   // MFMA kernel with the mass post-multiplication (the linear operators'
   // adjoint): out_c = post_sign[c] * |J| * M_ref * rhs_c per element
   const double* post_tiles;  // MFMA A tiles of M_ref: [mblock][kstep][64]

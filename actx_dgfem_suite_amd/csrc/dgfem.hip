@@ -284,13 +284,14 @@ MixedKernel pick_mixed(int prec, int eq, int order, bool stage = false) {
 }
 
 // The FP64 kernel "auto" picks, from measurements on MI355X at 3-4M DOFs
-// per field: the MFMA kernel everywhere except Maxwell P2, where the LDS-staged
-// kernel is 20% faster (Np = 10 fills one 16-row MFMA block to 62%, and the 18
-// reference-derivative contractions per element hold the MFMA kernel at one
-// wave per SIMD).  Elsewhere the MFMA kernel leads by 10-60% (Maxwell P4:
-// 324 vs 360 us, wave P3: 112 vs 173 us).
+// per field: the MFMA kernel.  It leads the LDS-staged kernel by 10-60%
+// (wave P3: 112 vs 173 us); at Maxwell P2 it does so only with the direct
+// flux, which frees the LDS for three waves per SIMD (79 against 88 us; with
+// the chunked producer it held one wave per SIMD and was 11% slower).
 int auto_variant(int equation, int order) {
-  return (equation == DG_EQ_MAXWELL && order == 2) ? DG_KERNEL_LDS : DG_KERNEL_MFMA;
+  (void)equation;
+  (void)order;
+  return DG_KERNEL_MFMA;
 }
 
 // {{{ locality schedule
@@ -632,8 +633,10 @@ __global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
 
 // }}}
 
-// A-operand tiles of the MFMA kernel (see rhs_mfma.h for the layout).
-void build_tiles(int np, int nfp, const double* op, const double* lift,
+// A-operand tiles of the MFMA kernel (see rhs_mfma.h for the layout).  With
+// the direct flux the LIFT tiles take its k order: k-step ks is face
+// ks / kpf, lane group g its node 4 (ks % kpf) + g (zero past Nfp).
+void build_tiles(int np, int nfp, const double* op, const double* lift, bool direct,
                  std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
   const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, nfn = kFaces * nfp;
   // A row of lane l in block b: l % 16 for v_mfma_f64_16x16x4f64, l % 4 for
@@ -651,6 +654,18 @@ void build_tiles(int np, int nfp, const double* op, const double* lift,
           if (m < np && j < np)
             op_tiles[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] = op[((size_t)r * np + m) * np + j];
         }
+  if (direct) {
+    const int kpf = (nfp + 3) / 4, ksd = kFaces * kpf;
+    lift_tiles.assign((size_t)mb * ksd * 64, 0.0);
+    for (int b = 0; b < mb; ++b)
+      for (int ks = 0; ks < ksd; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int m = row(b, l), node = 4 * (ks % kpf) + (l >> 4);
+          if (m < np && node < nfp)
+            lift_tiles[((size_t)b * ksd + ks) * 64 + l] = lift[(size_t)m * nfn + (ks / kpf) * nfp + node];
+        }
+    return;
+  }
   for (int b = 0; b < mb; ++b)
     for (int ks = 0; ks < nfp; ++ks)
       for (int l = 0; l < 64; ++l) {
@@ -712,6 +727,23 @@ void build_item_map(int64_t k, int order, int c, int nchunk, const int32_t* emap
     }
 }
 
+// The exterior map in the direct flux order of the MFMA kernel:
+//   ext_d[t][ks][l] = emap[16 t + l % 16][(ks / kpf) * Nfp + 4 (ks % kpf) + l / 16]
+// (0, a valid node, past Nfp; slots past the last element repeat it).
+void build_direct_map(int64_t k, int order, const int32_t* emap, std::vector<int32_t>& ext_d) {
+  const int nfp = nfp_of(order), nfn = kFaces * nfp;
+  const int kpf = (nfp + 3) / 4, ksd = kFaces * kpf;
+  const int64_t ntiles = (k + 15) / 16;
+  ext_d.assign((size_t)ntiles * ksd * 64, 0);
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int ks = 0; ks < ksd; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int node = 4 * (ks % kpf) + (l >> 4);
+        const int64_t e = std::min<int64_t>(t * 16 + (l & 15), k - 1);
+        if (node < nfp) ext_d[((size_t)t * ksd + ks) * 64 + l] = emap[e * nfn + (ks / kpf) * nfp + node];
+      }
+}
+
 }  // namespace
 
 struct dg_plan {
@@ -744,6 +776,7 @@ struct dg_plan {
   double* vgeo = nullptr;
   double* fgeo = nullptr;
   int32_t* ext_t = nullptr;
+  int32_t* ext_d = nullptr;    // direct flux order, where the kernel uses it
   // mixed-precision forms: A tiles and launch grid per precision
   uint32_t* mixed_op[2] = {nullptr, nullptr};
   uint32_t* mixed_lift[2] = {nullptr, nullptr};
@@ -794,6 +827,7 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->vgeo);
   (void)hipFree(p->fgeo);
   (void)hipFree(p->ext_t);
+  (void)hipFree(p->ext_d);
   for (int k = 0; k < 2; ++k) {
     (void)hipFree(p->mixed_op[k]);
     (void)hipFree(p->mixed_lift[k]);
@@ -887,6 +921,7 @@ RhsArgs rhs_args(const dg_plan* p, const double* const* in, double* const* out, 
   a.vgeo = p->vgeo;
   a.fgeo = p->fgeo;
   a.ext_t = p->ext_t;
+  a.ext_d = p->ext_d;
   a.nelem = p->nelem;
   a.p0 = p->params[0];
   return a;
@@ -1144,7 +1179,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     for (int fn = 0; fn < nfn; ++fn) lt[fn * np + i] = d->lift[i * nfn + fn];
   if (!tri) {
     std::vector<double> op_tiles, lift_tiles;
-    build_tiles(np, nfp, dsrc, d->lift, op_tiles, lift_tiles);
+    build_tiles(np, nfp, dsrc, d->lift, direct_flux_of(d->equation, d->order), op_tiles, lift_tiles);
     if ((rc = upload(&p->op_tiles, op_tiles.data(), op_tiles.size())) ||
         (rc = upload(&p->lift_tiles, lift_tiles.data(), lift_tiles.size()))) {
       free_plan(p);
@@ -1167,6 +1202,14 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
         (rc = upload(&p->ext_t, ext_t.data(), ext_t.size()))) {
       free_plan(p);
       return rc;
+    }
+    if (direct_flux_of(d->equation, d->order)) {
+      std::vector<int32_t> ext_d;
+      build_direct_map(k, d->order, d->ext_map, ext_d);
+      if ((rc = upload(&p->ext_d, ext_d.data(), ext_d.size()))) {
+        free_plan(p);
+        return rc;
+      }
     }
   }
   if ((rc = upload(&p->dmat_t, dt.data(), dt.size())) ||
@@ -1464,9 +1507,9 @@ void adjoint_reflection(const dg_plan* p, double* sign) {
 // mass goes into the workspace; then, for an MFMA plan, one launch of the MFMA
 // kernel with R |J| M_ref fused into its output phase writes q_bar, and for an
 // LDS-staged plan the forward kernel writes q_bar and a mass pass scales it in
-// place (the MFMA variant at Maxwell P2 is slower than both).  Measured on
-// MI355X, against the dedicated adjoint kernel: wave P3 259 -> 156 us, wave
-// P4 366 -> 220 us, Maxwell P4 711 -> 439 us, Maxwell P2 (LDS) 272 -> 274 us.
+// place.  Measured on MI355X, against the dedicated adjoint kernel: wave P3
+// 259 -> 156 us, wave P4 366 -> 220 us, Maxwell P4 711 -> 439 us; Maxwell P2
+// (162k tets) 122 us on the MFMA kernel, 158 us on the LDS-staged one.
 //
 // Euler, triangles and the FP32 adjoint: mu = M_ref^{-1} lam into the
 // workspace, then the dedicated adjoint kernel (on triangles the two mass

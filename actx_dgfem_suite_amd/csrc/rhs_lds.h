@@ -1,6 +1,6 @@
 // Fused RHS kernel with one thread per (element, node), element batches and
 // the transposed operators staged in LDS: the baseline the MFMA kernel is
-// measured against, and the "auto" choice where it is faster (Maxwell P2).
+// measured against (plan variant "lds").
 #pragma once
 
 #include "dg_common.h"

@@ -82,8 +82,11 @@ struct MfmaShape {
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks of the output
   static constexpr int KS = (NP + 3) / 4;     // k-steps of one D_r contraction
   static constexpr int KSL = NFP;             // k-steps of the LIFT contraction
+  static constexpr int KPF = (NFP + 3) / 4;   // direct flux: k-steps per face
+  static constexpr int KSD = kFaces * KPF;    // direct flux: LIFT k-steps
   static constexpr int OP_TILES = 3 * MB * KS;
   static constexpr int LIFT_TILES = MB * KSL;
+  static constexpr int LIFT_TILES_D = MB * KSD;
 };
 
 // Whether a wave loads the exterior-map entries of its next flux chunk ahead
@@ -121,6 +124,26 @@ __host__ __device__ constexpr int flux_items_of(int eq, int order) {
          flux_chunk_of(eq, order) * 64;
 }
 
+// Whether the surface phase evaluates the flux directly in the LIFT B-operand
+// layout: k-step ks covers face f = ks / KPF and its nodes 4 (ks % KPF) + g for
+// the tile's 16 elements (lane group g, lane n = element), so each lane forms
+// the flux value its own B operand needs, the LIFT A tiles follow that k order
+// (KPF = ceil(Nfp / 4) k-steps per face, the nodes past Nfp zero-padded), and
+// no flux goes through LDS.  The exterior map comes in the same order (ext_d).
+// One gather instruction then reads 4 nodes of one face of each of 16
+// neighbours.  It frees the flux buffer's LDS, which pays for the staged own
+// traces at wave P3 and for a third wave per SIMD at Maxwell P2.  Measured on
+// MI355X against the chunked producer: wave P3 -2% (162k tets) to -8% (384k),
+// Maxwell P2 79 against 88 us for the LDS-staged kernel, Maxwell P4 -3%;
+// slower elsewhere (Euler +20%, where the padded face nodes cost a third more
+// flux evaluations; wave P2 +6%, wave P4 +5%; three waves per SIMD at wave P3
+// without the staged traces +10% to +40%).
+__host__ __device__ constexpr bool direct_flux_of(int eq, int order) {
+  return (eq == DG_EQ_WAVE && order == 3) || (eq == DG_EQ_MAXWELL && (order == 2 || order == 4));
+}
+template <int EQ, int ORDER>
+constexpr bool direct_flux() { return direct_flux_of(EQ, ORDER); }
+
 // Whether the volume phase keeps a copy of the tile's state in LDS (as it
 // loads the B operands) so that the flux producer reads its own traces from
 // there instead of issuing NF more global loads per flux item.  It costs
@@ -130,6 +153,7 @@ __host__ __device__ constexpr int flux_items_of(int eq, int order) {
 // where the LDS halves them (wave P3 +8%, Euler P2 +18%); at P4 it does not fit.
 template <int EQ, int ORDER>
 constexpr bool stage_own_traces() {
+  if constexpr (direct_flux<EQ, ORDER>()) return ORDER <= 3;
   if constexpr (EQ == DG_EQ_WAVE) return ORDER <= 2;
   if constexpr (EQ == DG_EQ_EULER) return ORDER == 3;
   return ORDER <= 3;
@@ -163,11 +187,14 @@ constexpr bool branch_free_gathers() {
 
 constexpr int kFluxRow = 65;                 // padded flux row (spreads LDS banks)
 
-// Per-wave LDS buffer (doubles): one flux chunk, or one field of the output tile.
+// Per-wave LDS buffer (doubles): one flux chunk, or one field of the output
+// tile.  With the direct flux and the staged state, the output tile goes
+// through the stage (dead after the surface phase) and there is no buffer.
 template <int EQ, int ORDER>
 constexpr int wave_buffer() {
-  constexpr int flux = flux_chunk<EQ, ORDER>() * nfields_of(EQ) * kFluxRow;
+  constexpr int flux = direct_flux<EQ, ORDER>() ? 0 : flux_chunk<EQ, ORDER>() * nfields_of(EQ) * kFluxRow;
   constexpr int out = 16 * np_of(ORDER);
+  if constexpr (direct_flux<EQ, ORDER>() && stage_own_traces<EQ, ORDER>()) return 1;
   return flux > out ? flux : out;
 }
 
@@ -186,6 +213,10 @@ __device__ constexpr int levi(int o, int x, int a) {
 // Unroll depth of the volume k-step loops: enough independent MFMAs and loads
 // in flight without hoisting every load of the tile into registers.
 #define KUNROLL 3
+// Unroll depth of the direct-flux k-step loop (measured: 3 against 2 and 4,
+// except Maxwell P4, 4).
+#define DUNROLL 3
+#define DUNROLL_P4 4
 
 // POST: the kernel multiplies each element's RHS by post_sign_c |J| M_ref
 // before storing it, one more contraction per field with the output tile as
@@ -205,13 +236,15 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
   constexpr int BLOCK = 64 * WAVES;
 
   constexpr bool SH = short_last_block(NP);   // last output block on 4x4x4
+  constexpr bool DF = direct_flux<EQ, ORDER>();
+  constexpr int LT = DF ? S::LIFT_TILES_D : S::LIFT_TILES;
   constexpr int C = flux_chunk<EQ, ORDER>();
   constexpr int NCHUNK = (NFP + C - 1) / C;
   constexpr int WB = wave_buffer<EQ, ORDER>();
   constexpr int FROW = kFluxRow;
 
   __shared__ double s_op[S::OP_TILES * 64];
-  __shared__ double s_lift[S::LIFT_TILES * 64];
+  __shared__ double s_lift[LT * 64];
   __shared__ double s_post[POST ? MB * KS * 64 : 1];
   __shared__ int s_fmask[NFN];
   // per wave: scaled flux of C LIFT k-steps in MFMA B order [kk][field][FROW],
@@ -221,14 +254,16 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
   constexpr bool ST = stage_own_traces<EQ, ORDER>();
   __shared__ double s_stage[WAVES][ST ? NF * 16 * NP : 1];
   double* const stg = s_stage[threadIdx.x >> 6];
-  // per wave, if enabled: the tile's face geometry [face * 16 + n][4]
-  constexpr bool FG = face_geometry_in_lds<EQ, ORDER>();
+  // per wave, if enabled: the tile's face geometry [face * 16 + n][4] (not
+  // with the post-multiplication on top of the staged state, where its 8 KB
+  // would cost a block per CU at wave P3)
+  constexpr bool FG = face_geometry_in_lds<EQ, ORDER>() && !(POST && DF && ST);
   __shared__ double2 s_fgeo[WAVES][FG ? 128 : 1];
   double2* const fgl = s_fgeo[threadIdx.x >> 6];
 
   const int tid = threadIdx.x;
   for (int k = tid; k < S::OP_TILES * 64; k += BLOCK) s_op[k] = a.op_tiles[k];
-  for (int k = tid; k < S::LIFT_TILES * 64; k += BLOCK) s_lift[k] = a.lift_tiles[k];
+  for (int k = tid; k < LT * 64; k += BLOCK) s_lift[k] = a.lift_tiles[k];
   for (int k = tid; k < NFN; k += BLOCK) s_fmask[k] = a.fmask[k];
   if constexpr (POST)
     for (int k = tid; k < MB * KS * 64; k += BLOCK) s_post[k] = a.post_tiles[k];
@@ -243,12 +278,12 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
   const int nslots = blocks_per_xcd() * WAVES;
   const int32_t p_end = a.sched.part[block_xcd() + 1];
   const double c0 = a.p0;
-  double* const buf = s_buf[tid >> 6];
+  double* const buf = (DF && ST) ? stg : s_buf[tid >> 6];
 
   // exterior-map entries of one flux chunk's items, loaded one chunk ahead
   // (the first chunk's before the volume phase) so that the gathers do not
   // wait on the map
-  constexpr bool PF = prefetch_ext<EQ, ORDER>();
+  constexpr bool PF = !DF && prefetch_ext<EQ, ORDER>();
   int32_t ext_pf[C];
   auto load_ext = [&](int64_t t, int ch) {
 #pragma unroll
@@ -422,10 +457,57 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 
     // }}}
 
-    // {{{ surface terms, C LIFT k-steps per chunk
+    // {{{ surface terms
 
     if constexpr (ST || FG) wave_lds_fence();   // staged state / face geometry complete
 
+    if constexpr (DF) {
+      // direct: k-step ks, lane (g, n) evaluates the flux at node 4 (ks % KPF)
+      // + g of face ks / KPF of element n and feeds it to the LIFT MFMAs
+      const int32_t* const ext_row = a.ext_d + tile * (S::KSD * 64) + lane;
+#pragma unroll(ORDER == 4 ? DUNROLL_P4 : DUNROLL)
+      for (int ks = 0; ks < S::KSD; ++ks) {
+        const int f = ks / S::KPF;
+        const int node = 4 * (ks % S::KPF) + g;
+        const bool fok = node < NFP;
+        const int vloc = s_fmask[f * NFP + (fok ? node : 0)];
+        const int64_t vn = el * NP + vloc;
+        const double2* pgeo = FG ? fgl + 2 * (f * 16 + n)
+                                 : reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + f * 16 + n) * 4);
+        const double2 g01 = pgeo[0], g23 = pgeo[1];
+        const double nrm[3] = {g01.x, g01.y, g23.x};
+        double qm[NF], qp[NF], out[NF];
+        if constexpr (ST) {
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * NP + n * NP + vloc];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
+        }
+        // padded nodes read entry 0 (a valid node) and contribute nothing
+        const int32_t ext = ext_row[ks * 64];
+        const int64_t src = ext >= 0 ? (int64_t)ext : vn;
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][src];
+        double qb[NF];
+        Flux::boundary(qm, nrm, qb);
+#pragma unroll
+        for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+        Flux::flux(qm, qp, nrm, c0, out);
+        double bv[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) bv[c] = fok ? g23.y * out[c] : 0.0;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+          const double av = s_lift[(b * S::KSD + ks) * 64 + lane];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) mma(acc[c][b], av, bv[c], SH && b == MB - 1);
+        }
+      }
+      // the output tile goes through the stage (buf == stg): every lane's
+      // reads of the state are done
+      if constexpr (ST) wave_lds_fence();
+    } else {
 #pragma unroll 1
     for (int ch = 0; ch < NCHUNK; ++ch) {
       // producer: items (face pair p, node kk) with kk fastest
@@ -495,6 +577,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
         }
       }
       wave_lds_fence();
+    }
     }
 
     // }}}

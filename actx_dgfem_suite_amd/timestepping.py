@@ -72,10 +72,10 @@ _FUSED_MIXED = {
 
 # Linear operators whose fused reverse step in the adjoint variables
 # (``dg_rhs_lserk_stage_adjoint``) measured at least 3% faster on MI355X
-# than ``dg_rhs`` + ``dg_lserk_stage_adjoint`` (10 reverse steps, meshes of
-# 94k-750k tets): wave P1 1.10x, P2 1.07x, P4 1.03x.  Wave P3 0.98x, Maxwell
-# P1 0.99x, P3 1.01x, P4 0.98x; Maxwell P2 runs the LDS-staged kernel.
-_FUSED_REVERSE = {("wave", 1), ("wave", 2), ("wave", 4)}
+# than ``dg_rhs`` + ``dg_lserk_stage_adjoint`` (reverse steps on meshes of
+# 94k-750k tets): wave P1 1.10x, P2 1.07x, P3 1.13x, P4 1.03x, Maxwell P2
+# 1.03x.  Maxwell P1 0.99x, P3 1.01x, P4 0.87x.
+_FUSED_REVERSE = {("wave", 1), ("wave", 2), ("wave", 3), ("wave", 4), ("maxwell", 2)}
 
 
 def fused_stage_preferred(plan, precision: str = "fp64") -> bool:
