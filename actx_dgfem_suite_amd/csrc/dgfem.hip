@@ -288,6 +288,28 @@ MixedKernel pick_mixed(int prec, int eq, int order, bool stage = false) {
 // (wave P3: 112 vs 173 us); at Maxwell P2 it does so only with the direct
 // flux, which frees the LDS for three waves per SIMD (79 against 88 us; with
 // the chunked producer it held one wave per SIMD and was 11% slower).
+// Whether every element's face geometry is the affine image of the reference
+// element's: Fscale_f n_f = rx^T c_f with c_f = ref_face_coef (relative
+// 1e-10), which the MFMA kernel's reference-normal LIFT relies on.
+bool geometry_fits_ref_normals(int64_t k, const double* geo) {
+  for (int64_t e = 0; e < k; ++e) {
+    const double* g = geo + e * kGeo;
+    for (int f = 0; f < kFaces; ++f) {
+      const double fs = g[9 + 4 * f + 3];
+      for (int x = 0; x < 3; ++x) {
+        double v = 0.0, mag = 0.0;
+        for (int r = 0; r < 3; ++r) {
+          v += g[3 * x + r] * ref_face_coef(f, r);
+          mag += std::fabs(g[3 * x + r]);
+        }
+        const double want = fs * g[9 + 4 * f + x];
+        if (!(std::fabs(v - want) <= 1e-10 * (mag + std::fabs(want)))) return false;
+      }
+    }
+  }
+  return true;
+}
+
 int auto_variant(int equation, int order) {
   (void)equation;
   (void)order;
@@ -634,10 +656,12 @@ __global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
 // }}}
 
 // A-operand tiles of the MFMA kernel (see rhs_mfma.h for the layout).  With
-// the direct flux the LIFT tiles take its k order: k-step ks is face
-// ks / kpf, lane group g its node 4 (ks % kpf) + g (zero past Nfp).
-void build_tiles(int np, int nfp, const double* op, const double* lift, bool direct,
+// the direct flux the LIFT tiles take its k order: lane group g of k-step ks
+// holds face-node pair direct_pair_of(eq, order, ks, g) (zero for a padded
+// slot).
+void build_tiles(int eq, int order, const double* op, const double* lift, bool direct,
                  std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
+  const int np = np_of(order), nfp = nfp_of(order);
   const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, nfn = kFaces * nfp;
   // A row of lane l in block b: l % 16 for v_mfma_f64_16x16x4f64, l % 4 for
   // the 4x4x4 form a short last block runs on
@@ -655,14 +679,14 @@ void build_tiles(int np, int nfp, const double* op, const double* lift, bool dir
             op_tiles[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] = op[((size_t)r * np + m) * np + j];
         }
   if (direct) {
-    const int kpf = (nfp + 3) / 4, ksd = kFaces * kpf;
+    const int ksd = direct_ksteps_of(eq, order);
     lift_tiles.assign((size_t)mb * ksd * 64, 0.0);
     for (int b = 0; b < mb; ++b)
       for (int ks = 0; ks < ksd; ++ks)
         for (int l = 0; l < 64; ++l) {
-          const int m = row(b, l), node = 4 * (ks % kpf) + (l >> 4);
-          if (m < np && node < nfp)
-            lift_tiles[((size_t)b * ksd + ks) * 64 + l] = lift[(size_t)m * nfn + (ks / kpf) * nfp + node];
+          const int m = row(b, l), fp = direct_pair_of(eq, order, ks, l >> 4);
+          if (m < np && fp >= 0)
+            lift_tiles[((size_t)b * ksd + ks) * 64 + l] = lift[(size_t)m * nfn + fp];
         }
     return;
   }
@@ -728,19 +752,20 @@ void build_item_map(int64_t k, int order, int c, int nchunk, const int32_t* emap
 }
 
 // The exterior map in the direct flux order of the MFMA kernel:
-//   ext_d[t][ks][l] = emap[16 t + l % 16][(ks / kpf) * Nfp + 4 (ks % kpf) + l / 16]
-// (0, a valid node, past Nfp; slots past the last element repeat it).
-void build_direct_map(int64_t k, int order, const int32_t* emap, std::vector<int32_t>& ext_d) {
-  const int nfp = nfp_of(order), nfn = kFaces * nfp;
-  const int kpf = (nfp + 3) / 4, ksd = kFaces * kpf;
+//   ext_d[t][ks][l] = emap[16 t + l % 16][direct_pair_of(eq, order, ks, l / 16)]
+// (0, a valid node, for a padded slot; slots past the last element repeat it).
+void build_direct_map(int64_t k, int eq, int order, const int32_t* emap,
+                      std::vector<int32_t>& ext_d) {
+  const int nfn = kFaces * nfp_of(order);
+  const int ksd = direct_ksteps_of(eq, order);
   const int64_t ntiles = (k + 15) / 16;
   ext_d.assign((size_t)ntiles * ksd * 64, 0);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int ks = 0; ks < ksd; ++ks)
       for (int l = 0; l < 64; ++l) {
-        const int node = 4 * (ks % kpf) + (l >> 4);
+        const int fp = direct_pair_of(eq, order, ks, l >> 4);
         const int64_t e = std::min<int64_t>(t * 16 + (l & 15), k - 1);
-        if (node < nfp) ext_d[((size_t)t * ksd + ks) * 64 + l] = emap[e * nfn + (ks / kpf) * nfp + node];
+        if (fp >= 0) ext_d[((size_t)t * ksd + ks) * 64 + l] = emap[e * nfn + fp];
       }
 }
 
@@ -1160,6 +1185,16 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     p->kernel = {nullptr, kBlock, kBlock / np};
   } else {
     p->variant = d->variant == DG_KERNEL_AUTO ? auto_variant(d->equation, d->order) : d->variant;
+    if (p->variant == DG_KERNEL_MFMA && ref_normal_lift_of(d->equation, d->order) &&
+        !geometry_fits_ref_normals(d->nelements, d->geometry)) {
+      if (d->variant == DG_KERNEL_MFMA) {
+        free_plan(p);
+        return fail(DG_ERR_INVALID_ARGUMENT,
+                    "MFMA kernel: the face geometry is not the affine image of the reference "
+                    "element's (Fscale n != rx^T c_f); the LDS-staged kernel takes any geometry");
+      }
+      p->variant = DG_KERNEL_LDS;
+    }
     p->kernel = p->variant == DG_KERNEL_LDS ? pick_lds(d->equation, d->order)
                                             : pick_mfma(d->equation, d->order);
   }
@@ -1179,7 +1214,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     for (int fn = 0; fn < nfn; ++fn) lt[fn * np + i] = d->lift[i * nfn + fn];
   if (!tri) {
     std::vector<double> op_tiles, lift_tiles;
-    build_tiles(np, nfp, dsrc, d->lift, direct_flux_of(d->equation, d->order), op_tiles, lift_tiles);
+    build_tiles(d->equation, d->order, dsrc, d->lift, direct_flux_of(d->equation, d->order),
+                op_tiles, lift_tiles);
     if ((rc = upload(&p->op_tiles, op_tiles.data(), op_tiles.size())) ||
         (rc = upload(&p->lift_tiles, lift_tiles.data(), lift_tiles.size()))) {
       free_plan(p);
@@ -1205,7 +1241,7 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     }
     if (direct_flux_of(d->equation, d->order)) {
       std::vector<int32_t> ext_d;
-      build_direct_map(k, d->order, d->ext_map, ext_d);
+      build_direct_map(k, d->equation, d->order, d->ext_map, ext_d);
       if ((rc = upload(&p->ext_d, ext_d.data(), ext_d.size()))) {
         free_plan(p);
         return rc;

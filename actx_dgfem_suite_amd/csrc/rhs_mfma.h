@@ -82,11 +82,8 @@ struct MfmaShape {
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks of the output
   static constexpr int KS = (NP + 3) / 4;     // k-steps of one D_r contraction
   static constexpr int KSL = NFP;             // k-steps of the LIFT contraction
-  static constexpr int KPF = (NFP + 3) / 4;   // direct flux: k-steps per face
-  static constexpr int KSD = kFaces * KPF;    // direct flux: LIFT k-steps
   static constexpr int OP_TILES = 3 * MB * KS;
   static constexpr int LIFT_TILES = MB * KSL;
-  static constexpr int LIFT_TILES_D = MB * KSD;
 };
 
 // Whether a wave loads the exterior-map entries of its next flux chunk ahead
@@ -125,24 +122,73 @@ __host__ __device__ constexpr int flux_items_of(int eq, int order) {
 }
 
 // Whether the surface phase evaluates the flux directly in the LIFT B-operand
-// layout: k-step ks covers face f = ks / KPF and its nodes 4 (ks % KPF) + g for
-// the tile's 16 elements (lane group g, lane n = element), so each lane forms
-// the flux value its own B operand needs, the LIFT A tiles follow that k order
-// (KPF = ceil(Nfp / 4) k-steps per face, the nodes past Nfp zero-padded), and
-// no flux goes through LDS.  The exterior map comes in the same order (ext_d).
-// One gather instruction then reads 4 nodes of one face of each of 16
-// neighbours.  It frees the flux buffer's LDS, which pays for the staged own
-// traces at wave P3 and for a third wave per SIMD at Maxwell P2.  Measured on
-// MI355X against the chunked producer: wave P3 -2% (162k tets) to -8% (384k),
-// Maxwell P2 79 against 88 us for the LDS-staged kernel, Maxwell P4 -3%;
+// layout: at k-step ks, lane (g, n) evaluates the flux at one face node of
+// element n (the face-node pair of lane group g, direct_pair_of below), so
+// each lane forms the flux value its own B operand needs, the LIFT A tiles
+// follow that k order and no flux goes through LDS.  The exterior map comes
+// in the same order (ext_d).  One gather instruction then reads 4 face nodes
+// of each of 16 neighbours.  It frees the flux buffer's LDS, which pays for
+// the staged own traces at wave P3 and for a third wave per SIMD at Maxwell
+// P2, and it is what lets the wave operator LIFT its velocity flux in
+// reference normals (ref_normal_lift).  Measured on MI355X against the
+// chunked producer: Maxwell P2 79 against 88 us for the LDS-staged kernel,
+// Maxwell P4 -3%; with the reference-normal LIFT, wave P2 -1.5%, P4 -2.3%;
 // slower elsewhere (Euler +20%, where the padded face nodes cost a third more
-// flux evaluations; wave P2 +6%, wave P4 +5%; three waves per SIMD at wave P3
-// without the staged traces +10% to +40%).
+// flux evaluations; wave P1 +15%).
 __host__ __device__ constexpr bool direct_flux_of(int eq, int order) {
-  return (eq == DG_EQ_WAVE && order == 3) || (eq == DG_EQ_MAXWELL && (order == 2 || order == 4));
+  return (eq == DG_EQ_WAVE && order >= 2) || (eq == DG_EQ_MAXWELL && (order == 2 || order == 4));
 }
 template <int EQ, int ORDER>
 constexpr bool direct_flux() { return direct_flux_of(EQ, ORDER); }
+
+// The direct flux's k order.  Padded: k-step ks covers face ks / KPF, lane
+// group g its node 4 (ks % KPF) + g (KPF = ceil(Nfp / 4) k-steps per face,
+// nodes past Nfp zero-padded).  Packed: k-step ks covers the face-node pair
+// p = 4 ks + g, i.e. face p / Nfp and node p % Nfp, so the 4 Nfp pairs fill
+// Nfp k-steps with no padding and a k-step may straddle two faces.  Packed
+// except where the straddling k-steps would cost the wave operator's
+// reference-normal LIFT (below) more MFMAs than the padding: wave P1, P4.
+__host__ __device__ constexpr bool packed_direct_of(int eq, int order) {
+  return eq != DG_EQ_WAVE || order == 2 || order == 3;
+}
+__host__ __device__ constexpr int direct_ksteps_of(int eq, int order) {
+  return packed_direct_of(eq, order) ? nfp_of(order) : kFaces * ((nfp_of(order) + 3) / 4);
+}
+// Face-node pair (face * Nfp + node) of k-step ks, lane group g, or -1 for a
+// padded slot.
+__host__ __device__ constexpr int direct_pair_of(int eq, int order, int ks, int g) {
+  return packed_direct_of(eq, order)
+             ? 4 * ks + g
+             : (4 * (ks % ((nfp_of(order) + 3) / 4)) + g < nfp_of(order)
+                    ? (ks / ((nfp_of(order) + 3) / 4)) * nfp_of(order) +
+                          4 * (ks % ((nfp_of(order) + 3) / 4)) + g
+                    : -1);
+}
+
+// Whether the wave operator LIFTs its velocity flux in reference normals.
+// The upwind flux's velocity part is s n (s scalar per face node), and on an
+// affine tetrahedron Fscale_f n_f = rx^T c_f (Nanson's formula), where
+// c_f = the reference face's outward normal times its area ratio is
+// (0, 0, -1), (0, -1, 0), (1, 1, 1), (-1, 0, 0) for faces 0..3 (the plan
+// checks every element's geometry against it).  So
+//   LIFT(Fscale s n_x) = sum_r rx[x][r] sum_f c_{f,r} LIFT_f s_f,
+// and the velocity flux is LIFTed into the per-r gradient accumulators
+// t_r = D_r u before their combination with rx: one MFMA per k-step on an
+// axis face and three on the slanted face, against three (one per velocity
+// field) before.  With the direct flux layout (one face per lane group, or
+// two in a straddling packed k-step) the MFMAs a k-step needs are known at
+// compile time.  Measured on MI355X: wave P3 -10%.
+__host__ __device__ constexpr bool ref_normal_lift_of(int eq, int order) {
+  return eq == DG_EQ_WAVE && direct_flux_of(eq, order);
+}
+template <int EQ, int ORDER>
+constexpr bool ref_normal_lift() { return ref_normal_lift_of(EQ, ORDER); }
+
+// c_{f,r}: reference face normal times area ratio (faces of the reference
+// element in its face numbering).
+__host__ __device__ constexpr int ref_face_coef(int f, int r) {
+  return f == 2 ? 1 : ((f == 0 && r == 2) || (f == 1 && r == 1) || (f == 3 && r == 0) ? -1 : 0);
+}
 
 // Whether the volume phase keeps a copy of the tile's state in LDS (as it
 // loads the B operands) so that the flux producer reads its own traces from
@@ -237,7 +283,9 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 
   constexpr bool SH = short_last_block(NP);   // last output block on 4x4x4
   constexpr bool DF = direct_flux<EQ, ORDER>();
-  constexpr int LT = DF ? S::LIFT_TILES_D : S::LIFT_TILES;
+  constexpr bool RN = ref_normal_lift<EQ, ORDER>();
+  constexpr int KSD = direct_ksteps_of(EQ, ORDER);   // direct flux: LIFT k-steps
+  constexpr int LT = DF ? MB * KSD : S::LIFT_TILES;
   constexpr int C = flux_chunk<EQ, ORDER>();
   constexpr int NCHUNK = (NFP + C - 1) / C;
   constexpr int WB = wave_buffer<EQ, ORDER>();
@@ -291,8 +339,21 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
       ext_pf[it] = a.ext_t[t * flux_items_of(EQ, ORDER) + (ch * C + it) * 64 + lane];
   };
 
-  for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
-    const int64_t tile = a.sched.order[pos];
+  // the next tile's index is loaded while this tile runs
+  int32_t pos = a.sched.part[block_xcd()] + slot;
+  int32_t next_tile = pos < p_end ? a.sched.order[pos] : 0;
+  for (; pos < p_end; pos += nslots) {
+    const int64_t tile = next_tile;
+    if (pos + nslots < p_end) next_tile = a.sched.order[pos + nslots];
+    // direct flux: every k-step's exterior-map entry, loaded with the volume
+    // operands so that no gather waits on the map (wave: -1.5%; it costs
+    // registers that Maxwell P4 cannot spare, +9%)
+    constexpr bool MAP0 = DF && EQ == DG_EQ_WAVE;
+    int32_t ext_all[MAP0 ? KSD : 1];
+    if constexpr (MAP0) {
+#pragma unroll
+      for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
+    }
     if constexpr (FG) {
       const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
       const double2 f0 = src[lane], f1 = src[lane + 64];
@@ -315,9 +376,12 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 
     // {{{ volume terms
 
+    // wave: the gradient's per-r accumulators t_r = D_r u (with the
+    // reference-normal LIFT they take the velocity flux too, and are combined
+    // with rx after the surface phase)
+    f64x4 t[EQ == DG_EQ_WAVE ? 3 : 1][MB];
     if constexpr (EQ == DG_EQ_WAVE) {
       // u_t = -c sum_r D_r w_r (w_r = sum_x rx[x][r] v_x); v_x,t = -c sum_r rx[x][r] D_r u
-      f64x4 t[3][MB];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -355,10 +419,15 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
         for (int v = 0; v < ncomp(SH && b == MB - 1); ++v) {
           acc[0][b][v] *= -c0;
+          if constexpr (RN) {
 #pragma unroll
-          for (int x = 0; x < 3; ++x)
-            acc[1 + x][b][v] = -c0 * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
-                                      rx[3 * x + 2] * t[2][b][v]);
+            for (int r = 0; r < 3; ++r) t[r][b][v] *= -c0;
+          } else {
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+              acc[1 + x][b][v] = -c0 * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
+                                        rx[3 * x + 2] * t[2][b][v]);
+          }
         }
     } else if constexpr (EQ == DG_EQ_EULER) {
       // q_t = sum_r D_r w_{c,r},  w_{c,r} = sum_x rx[x][r] F_{c,x}(q)
@@ -462,21 +531,21 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
     if constexpr (ST || FG) wave_lds_fence();   // staged state / face geometry complete
 
     if constexpr (DF) {
-      // direct: k-step ks, lane (g, n) evaluates the flux at node 4 (ks % KPF)
-      // + g of face ks / KPF of element n and feeds it to the LIFT MFMAs
-      const int32_t* const ext_row = a.ext_d + tile * (S::KSD * 64) + lane;
-#pragma unroll(ORDER == 4 ? DUNROLL_P4 : DUNROLL)
-      for (int ks = 0; ks < S::KSD; ++ks) {
-        const int f = ks / S::KPF;
-        const int node = 4 * (ks % S::KPF) + g;
-        const bool fok = node < NFP;
-        const int vloc = s_fmask[f * NFP + (fok ? node : 0)];
-        const int64_t vn = el * NP + vloc;
+      // direct: at k-step ks, lane (g, n) evaluates the flux at face-node pair
+      // direct_pair_of(ks, g) of element n and feeds it to the LIFT MFMAs.
+      // One k-step's flux: pair fp (padded slots read the face's node 0, a
+      // valid node, and contribute nothing), its face f, and the state traces.
+      auto trace_pair = [&](int ks, int& f, bool& fok, int64_t& vn, double2& g01, double2& g23,
+                            double* qm, double* qp) {
+        const int fp = direct_pair_of(EQ, ORDER, ks, g);
+        fok = fp >= 0;
+        f = fok ? fp / NFP : ks / ((NFP + 3) / 4);
+        const int vloc = s_fmask[fok ? fp : f * NFP];
+        vn = el * NP + vloc;
         const double2* pgeo = FG ? fgl + 2 * (f * 16 + n)
                                  : reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + f * 16 + n) * 4);
-        const double2 g01 = pgeo[0], g23 = pgeo[1];
-        const double nrm[3] = {g01.x, g01.y, g23.x};
-        double qm[NF], qp[NF], out[NF];
+        g01 = pgeo[0];
+        g23 = pgeo[1];
         if constexpr (ST) {
 #pragma unroll
           for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * NP + n * NP + vloc];
@@ -484,24 +553,95 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
           for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
         }
-        // padded nodes read entry 0 (a valid node) and contribute nothing
-        const int32_t ext = ext_row[ks * 64];
+        int32_t ext;
+        if constexpr (MAP0) ext = ext_all[ks];
+        else ext = a.ext_d[(tile * KSD + ks) * 64 + lane];
         const int64_t src = ext >= 0 ? (int64_t)ext : vn;
 #pragma unroll
         for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][src];
         double qb[NF];
+        const double nrm[3] = {g01.x, g01.y, g23.x};
         Flux::boundary(qm, nrm, qb);
 #pragma unroll
         for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
-        Flux::flux(qm, qp, nrm, c0, out);
-        double bv[NF];
+      };
+      if constexpr (RN) {
+        // wave, velocity flux LIFTed in reference normals into t_r (every
+        // k-step unrolled, so the faces -- and the t_r -- a k-step touches
+        // are known at compile time)
 #pragma unroll
-        for (int c = 0; c < NF; ++c) bv[c] = fok ? g23.y * out[c] : 0.0;
+        for (int ks = 0; ks < KSD; ++ks) {
+          int f;
+          bool fok;
+          int64_t vn;
+          double2 g01, g23;
+          double qm[4], qp[4];
+          trace_pair(ks, f, fok, vn, g01, g23, qm, qp);
+          const double nrm[3] = {g01.x, g01.y, g23.x};
+          const double uavg = 0.5 * (qm[0] + qp[0]);
+          const double udiff = qp[0] - qm[0];
+          const double vavg_n = 0.5 * ((qm[1] + qp[1]) * nrm[0] + (qm[2] + qp[2]) * nrm[1] +
+                                       (qm[3] + qp[3]) * nrm[2]);
+          const double vjump = (qp[1] - qm[1]) * nrm[0] + (qp[2] - qm[2]) * nrm[1] +
+                               (qp[3] - qm[3]) * nrm[2];
+          // u: Fscale * the flux's u part; v: s = c ({u} + [v].n / 2), with
+          // c_{f,r} of this lane's face applied per t_r below
+          const double bu = fok ? g23.y * (c0 * (vavg_n + 0.5 * udiff)) : 0.0;
+          const double bs = fok ? c0 * (uavg + 0.5 * vjump) : 0.0;
+          // faces this k-step covers, over its four lane groups
+          constexpr int FPK = (NFP + 3) / 4;
+          const int f_lo = direct_pair_of(EQ, ORDER, ks, 0) >= 0
+                               ? direct_pair_of(EQ, ORDER, ks, 0) / NFP : ks / FPK;
+          int f_hi = f_lo;
 #pragma unroll
-        for (int b = 0; b < MB; ++b) {
-          const double av = s_lift[(b * S::KSD + ks) * 64 + lane];
+          for (int gg = 1; gg < 4; ++gg)
+            if (direct_pair_of(EQ, ORDER, ks, gg) >= 0)
+              f_hi = direct_pair_of(EQ, ORDER, ks, gg) / NFP;
 #pragma unroll
-          for (int c = 0; c < NF; ++c) mma(acc[c][b], av, bv[c], SH && b == MB - 1);
+          for (int b = 0; b < MB; ++b) {
+            const bool sh = SH && b == MB - 1;
+            const double av = s_lift[(b * KSD + ks) * 64 + lane];
+            mma(acc[0][b], av, bu, sh);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              bool any = false;
+#pragma unroll
+              for (int ff = 0; ff < 4; ++ff) any |= ff >= f_lo && ff <= f_hi && ref_face_coef(ff, r) != 0;
+              if (any) {
+                const double cs = (double)ref_face_coef(f, r) * bs;
+                mma(t[r][b], av, cs, sh);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+          for (int v = 0; v < ncomp(SH && b == MB - 1); ++v)
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+              acc[1 + x][b][v] = rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
+                                 rx[3 * x + 2] * t[2][b][v];
+      } else {
+#pragma unroll(ORDER == 4 ? DUNROLL_P4 : DUNROLL)
+        for (int ks = 0; ks < KSD; ++ks) {
+          int f;
+          bool fok;
+          int64_t vn;
+          double2 g01, g23;
+          double qm[NF], qp[NF], out[NF];
+          trace_pair(ks, f, fok, vn, g01, g23, qm, qp);
+          const double nrm[3] = {g01.x, g01.y, g23.x};
+          Flux::flux(qm, qp, nrm, c0, out);
+          double bv[NF];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) bv[c] = fok ? g23.y * out[c] : 0.0;
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const double av = s_lift[(b * KSD + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < NF; ++c) mma(acc[c][b], av, bv[c], SH && b == MB - 1);
+          }
         }
       }
       // the output tile goes through the stage (buf == stg): every lane's

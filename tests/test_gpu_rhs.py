@@ -399,3 +399,38 @@ def test_rhs_single_partial_tile(equation, order, variant, device):
             torch.cuda.synchronize()
             m = np.stack([t.cpu().numpy().astype(np.float64) for t in mixed])
             assert max(field_errors(m, ref)) < 1e-3, prec
+
+
+def test_mfma_wave_plan_checks_face_geometry(device):
+    """The MFMA wave kernel LIFTs the velocity flux in reference normals,
+    which holds when every element's face geometry is the affine image of
+    the reference element's (Fscale n_f = rx^T c_f).  A geometry that breaks
+    it is refused by an explicit MFMA plan and sent to the LDS-staged kernel
+    by an automatic one, which still matches the oracle on it."""
+    print("[case] wave P3 MFMA geometry check", flush=True)
+    d = make_box_discretization(3, 3)
+    geo = d.packed_geometry().copy()
+    geo[7, 9 + 4 * 2 + 3] *= 1.5          # one face's Fscale off the identity
+
+    class Bent:
+        """The same discretization with the altered face geometry."""
+        def __getattr__(self, name):
+            return getattr(d, name)
+
+        def packed_geometry(self):
+            return geo
+
+    bent = Bent()
+    with pytest.raises(ValueError, match="affine image"):
+        DGPlan(bent, "wave", 1.0, variant="mfma")
+    plan = DGPlan(bent, "wave", 1.0, variant="auto")
+    assert plan.kernel_variant == "lds"
+    # the unaltered discretization keeps the MFMA kernel
+    assert DGPlan(d, "wave", 1.0, variant="auto").kernel_variant == "mfma"
+    # the LDS-staged kernel evaluates the altered operator: compare it with a
+    # host evaluation of the same fused forms on the altered geometry
+    from oracle import dg_cpu
+    q = random_state("wave", d.nodes, seed=3)
+    got = _run(plan, q, device)
+    ref = dg_cpu.CpuOperator(bent, "wave", 1.0)(q)
+    assert rel_l2(got, ref) < TOL_FP64
