@@ -45,7 +45,7 @@ struct RhsArgs {
   const double* dmat_t;   // (3, Np(j), Np(i)): transposed weak or strong D
   const double* lift_t;   // (4*Nfp, Np(i))
   const int32_t* fmask;   // (4*Nfp)
-  const double* op_tiles;    // MFMA A tiles of D: [r][mblock][kstep][64]
+  const double* op_tiles;    // MFMA A tiles of D (wave: -c D): [r][mblock][kstep][64]
   const double* lift_tiles;  // MFMA A tiles of LIFT: [mblock][kstep][64]
   // MFMA kernel, tile-ordered copies of geometry and exterior map (16-element
   // tiles, the last one padded with its final element):
@@ -82,6 +82,8 @@ __device__ inline int blocks_per_xcd() { return gridDim.x >> 3; }
 // with [a] = exterior - interior; boundary state u+ = -u-, v+ = v- (:152-156).
 struct WaveFlux {
   static constexpr int NF = 4;
+  // the boundary state is the interior trace with per-field signs (u+ = -u-)
+  __host__ __device__ static constexpr int mirror_sign(int c) { return c == 0 ? -1 : 1; }
   __device__ static void boundary(const double* qm, const double* /*n*/, double* qp) {
     qp[0] = -qm[0];
     qp[1] = qm[1];
@@ -109,6 +111,8 @@ struct WaveFlux {
 // face_mass(F*), so out = -F*.  Field order: mass, energy, momentum[3].
 struct EulerFlux {
   static constexpr int NF = 5;
+  // the wall state depends on the normal: not a per-field sign
+  __host__ __device__ static constexpr int mirror_sign(int) { return 0; }
   __device__ static void boundary(const double* qm, const double* n, double* qp) {
     const double mn = qm[2] * n[0] + qm[3] * n[1] + qm[4] * n[2];
     qp[0] = qm[0];
@@ -150,6 +154,8 @@ struct EulerFlux {
 // PEC walls: E+ = -E-, H+ = H-.  The operator subtracts M^{-1} face_mass(flux).
 struct MaxwellFlux {
   static constexpr int NF = 6;
+  // PEC: E+ = -E-, H+ = H-
+  __host__ __device__ static constexpr int mirror_sign(int c) { return c < 3 ? -1 : 1; }
   __device__ static void boundary(const double* qm, const double* /*n*/, double* qp) {
     qp[0] = -qm[0];
     qp[1] = -qm[1];

@@ -660,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void lserk_mixed_kernel(LserkMixedArgs s) {
 // holds face-node pair direct_pair_of(eq, order, ks, g) (zero for a padded
 // slot).
 void build_tiles(int eq, int order, const double* op, const double* lift, bool direct,
-                 std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
+                 double op_scale, std::vector<double>& op_tiles, std::vector<double>& lift_tiles) {
   const int np = np_of(order), nfp = nfp_of(order);
   const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, nfn = kFaces * nfp;
   // A row of lane l in block b: l % 16 for v_mfma_f64_16x16x4f64, l % 4 for
@@ -676,7 +676,8 @@ void build_tiles(int eq, int order, const double* op, const double* lift, bool d
         for (int l = 0; l < 64; ++l) {
           const int m = row(b, l), j = (l >> 4) * ks_n + ks;
           if (m < np && j < np)
-            op_tiles[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] = op[((size_t)r * np + m) * np + j];
+            op_tiles[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] =
+                op_scale * op[((size_t)r * np + m) * np + j];
         }
   if (direct) {
     const int ksd = direct_ksteps_of(eq, order);
@@ -1214,8 +1215,9 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     for (int fn = 0; fn < nfn; ++fn) lt[fn * np + i] = d->lift[i * nfn + fn];
   if (!tri) {
     std::vector<double> op_tiles, lift_tiles;
+    // the wave kernel's D tiles carry the operator's factor -c
     build_tiles(d->equation, d->order, dsrc, d->lift, direct_flux_of(d->equation, d->order),
-                op_tiles, lift_tiles);
+                d->equation == DG_EQ_WAVE ? -d->params[0] : 1.0, op_tiles, lift_tiles);
     if ((rc = upload(&p->op_tiles, op_tiles.data(), op_tiles.size())) ||
         (rc = upload(&p->lift_tiles, lift_tiles.data(), lift_tiles.size()))) {
       free_plan(p);

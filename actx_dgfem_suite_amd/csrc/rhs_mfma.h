@@ -184,6 +184,18 @@ __host__ __device__ constexpr bool ref_normal_lift_of(int eq, int order) {
 template <int EQ, int ORDER>
 constexpr bool ref_normal_lift() { return ref_normal_lift_of(EQ, ORDER); }
 
+// Whether the flux producer takes a boundary state that is the interior trace
+// with per-field signs (wave, Maxwell PEC) by negating the loaded own trace
+// instead of forming Flux::boundary and selecting per field, and (wave with
+// the reference-normal LIFT) evaluates the upwind flux from the two normal
+// velocities.  Fewer VALU instructions: measured on MI355X wave P2 -6%, P3
+// -3.6%, Maxwell P3 -2%; at wave P4 (one wave per SIMD) each measured +2-3%,
+// so it keeps the general forms there.
+template <int EQ, int ORDER>
+constexpr bool lean_flux() {
+  return FluxOf<EQ>::type::mirror_sign(0) != 0 && !(EQ == DG_EQ_WAVE && ORDER == 4);
+}
+
 // c_{f,r}: reference face normal times area ratio (faces of the reference
 // element in its face numbering).
 __host__ __device__ constexpr int ref_face_coef(int f, int r) {
@@ -233,13 +245,19 @@ constexpr bool branch_free_gathers() {
 
 constexpr int kFluxRow = 65;                 // padded flux row (spreads LDS banks)
 
+// Row pitch (doubles) of one element's nodes in the per-wave LDS copies of a
+// tile (staged state, output tile): odd, so that the 16 elements a lane group
+// reads or writes at one node fall in distinct LDS banks (measured: Euler P2
+// -2%, wave P3 -1%), unless it would cost a block per CU.
+__host__ __device__ constexpr int stage_pitch(int np) { return np | 1; }
+
 // Per-wave LDS buffer (doubles): one flux chunk, or one field of the output
 // tile.  With the direct flux and the staged state, the output tile goes
 // through the stage (dead after the surface phase) and there is no buffer.
 template <int EQ, int ORDER>
-constexpr int wave_buffer() {
+constexpr int wave_buffer(int pitch) {
   constexpr int flux = direct_flux<EQ, ORDER>() ? 0 : flux_chunk<EQ, ORDER>() * nfields_of(EQ) * kFluxRow;
-  constexpr int out = 16 * np_of(ORDER);
+  const int out = 16 * pitch;
   if constexpr (direct_flux<EQ, ORDER>() && stage_own_traces<EQ, ORDER>()) return 1;
   return flux > out ? flux : out;
 }
@@ -288,8 +306,19 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
   constexpr int LT = DF ? MB * KSD : S::LIFT_TILES;
   constexpr int C = flux_chunk<EQ, ORDER>();
   constexpr int NCHUNK = (NFP + C - 1) / C;
-  constexpr int WB = wave_buffer<EQ, ORDER>();
   constexpr int FROW = kFluxRow;
+  constexpr bool ST = stage_own_traces<EQ, ORDER>();
+  constexpr bool FG = face_geometry_in_lds<EQ, ORDER>() && !(POST && DF && ST);
+  // LDS bytes of the kernel with element row pitch sp
+  constexpr auto lds_bytes = [](int sp) {
+    return 8 * (S::OP_TILES * 64 + LT * 64 + (POST ? MB * KS * 64 : 1)) + 4 * NFN +
+           8 * WAVES * (wave_buffer<EQ, ORDER>(sp) + (ST ? NF * 16 * sp : 1)) +
+           16 * WAVES * (FG ? 128 : 1);
+  };
+  // the odd pitch where it leaves the blocks per CU that the LDS allows
+  constexpr int SP = kLdsPerCu / lds_bytes(stage_pitch(NP)) == kLdsPerCu / lds_bytes(NP)
+                         ? stage_pitch(NP) : NP;
+  constexpr int WB = wave_buffer<EQ, ORDER>(SP);
 
   __shared__ double s_op[S::OP_TILES * 64];
   __shared__ double s_lift[LT * 64];
@@ -298,14 +327,12 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
   // per wave: scaled flux of C LIFT k-steps in MFMA B order [kk][field][FROW],
   // then one field of the output tile
   __shared__ double s_buf[WAVES][WB];
-  // per wave, if staged: the tile's state [field][n * NP + j]
-  constexpr bool ST = stage_own_traces<EQ, ORDER>();
-  __shared__ double s_stage[WAVES][ST ? NF * 16 * NP : 1];
+  // per wave, if staged: the tile's state [field][n * SP + j]
+  __shared__ double s_stage[WAVES][ST ? NF * 16 * SP : 1];
   double* const stg = s_stage[threadIdx.x >> 6];
   // per wave, if enabled: the tile's face geometry [face * 16 + n][4] (not
   // with the post-multiplication on top of the staged state, where its 8 KB
   // would cost a block per CU at wave P3)
-  constexpr bool FG = face_geometry_in_lds<EQ, ORDER>() && !(POST && DF && ST);
   __shared__ double2 s_fgeo[WAVES][FG ? 128 : 1];
   double2* const fgl = s_fgeo[threadIdx.x >> 6];
 
@@ -381,7 +408,8 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
     // with rx after the surface phase)
     f64x4 t[EQ == DG_EQ_WAVE ? 3 : 1][MB];
     if constexpr (EQ == DG_EQ_WAVE) {
-      // u_t = -c sum_r D_r w_r (w_r = sum_x rx[x][r] v_x); v_x,t = -c sum_r rx[x][r] D_r u
+      // u_t = -c sum_r D_r w_r (w_r = sum_x rx[x][r] v_x); v_x,t = -c sum_r rx[x][r] D_r u;
+      // the A tiles hold -c D_r
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -397,10 +425,10 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           vy = a.q.p[2][idx];
           vz = a.q.p[3][idx];
           if constexpr (ST) {
-            stg[0 * 16 * NP + n * NP + j] = u;
-            stg[1 * 16 * NP + n * NP + j] = vx;
-            stg[2 * 16 * NP + n * NP + j] = vy;
-            stg[3 * 16 * NP + n * NP + j] = vz;
+            stg[0 * 16 * SP + n * SP + j] = u;
+            stg[1 * 16 * SP + n * SP + j] = vx;
+            stg[2 * 16 * SP + n * SP + j] = vy;
+            stg[3 * 16 * SP + n * SP + j] = vz;
           }
         }
 #pragma unroll
@@ -414,21 +442,17 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           }
         }
       }
+      // (the host's D tiles of the wave operator carry the factor -c)
+      if constexpr (!RN) {
 #pragma unroll
-      for (int b = 0; b < MB; ++b)
+        for (int b = 0; b < MB; ++b)
 #pragma unroll
-        for (int v = 0; v < ncomp(SH && b == MB - 1); ++v) {
-          acc[0][b][v] *= -c0;
-          if constexpr (RN) {
-#pragma unroll
-            for (int r = 0; r < 3; ++r) t[r][b][v] *= -c0;
-          } else {
+          for (int v = 0; v < ncomp(SH && b == MB - 1); ++v)
 #pragma unroll
             for (int x = 0; x < 3; ++x)
-              acc[1 + x][b][v] = -c0 * (rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
-                                        rx[3 * x + 2] * t[2][b][v]);
-          }
-        }
+              acc[1 + x][b][v] = rx[3 * x] * t[0][b][v] + rx[3 * x + 1] * t[1][b][v] +
+                                 rx[3 * x + 2] * t[2][b][v];
+      }
     } else if constexpr (EQ == DG_EQ_EULER) {
       // q_t = sum_r D_r w_{c,r},  w_{c,r} = sum_x rx[x][r] F_{c,x}(q)
       const double gamma = c0;
@@ -441,10 +465,10 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           const double rho = a.q.p[0][idx], e = a.q.p[1][idx];
           const double m[3] = {a.q.p[2][idx], a.q.p[3][idx], a.q.p[4][idx]};
           if constexpr (ST) {
-            stg[0 * 16 * NP + n * NP + j] = rho;
-            stg[1 * 16 * NP + n * NP + j] = e;
+            stg[0 * 16 * SP + n * SP + j] = rho;
+            stg[1 * 16 * SP + n * SP + j] = e;
 #pragma unroll
-            for (int x = 0; x < 3; ++x) stg[(2 + x) * 16 * NP + n * NP + j] = m[x];
+            for (int x = 0; x < 3; ++x) stg[(2 + x) * 16 * SP + n * SP + j] = m[x];
           }
           const double inv_rho = 1.0 / rho;
           const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
@@ -494,7 +518,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           for (int c = 0; c < 6; ++c) f[c] = a.q.p[c][el * NP + j];
           if constexpr (ST)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) stg[c * 16 * NP + n * NP + j] = f[c];
+            for (int c = 0; c < 6; ++c) stg[c * 16 * SP + n * SP + j] = f[c];
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -548,7 +572,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
         g23 = pgeo[1];
         if constexpr (ST) {
 #pragma unroll
-          for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * NP + n * NP + vloc];
+          for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * SP + n * SP + vloc];
         } else {
 #pragma unroll
           for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
@@ -556,14 +580,22 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
         int32_t ext;
         if constexpr (MAP0) ext = ext_all[ks];
         else ext = a.ext_d[(tile * KSD + ks) * 64 + lane];
+        // a boundary node loads its own trace (src = vn), which is the
+        // boundary state up to the flux's per-field signs where it has them
         const int64_t src = ext >= 0 ? (int64_t)ext : vn;
 #pragma unroll
         for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][src];
-        double qb[NF];
-        const double nrm[3] = {g01.x, g01.y, g23.x};
-        Flux::boundary(qm, nrm, qb);
+        if constexpr (lean_flux<EQ, ORDER>()) {
 #pragma unroll
-        for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+          for (int c = 0; c < NF; ++c)
+            if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
+        } else {
+          double qb[NF];
+          const double nrm[3] = {g01.x, g01.y, g23.x};
+          Flux::boundary(qm, nrm, qb);
+#pragma unroll
+          for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+        }
       };
       if constexpr (RN) {
         // wave, velocity flux LIFTed in reference normals into t_r (every
@@ -577,17 +609,30 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           double2 g01, g23;
           double qm[4], qp[4];
           trace_pair(ks, f, fok, vn, g01, g23, qm, qp);
-          const double nrm[3] = {g01.x, g01.y, g23.x};
-          const double uavg = 0.5 * (qm[0] + qp[0]);
-          const double udiff = qp[0] - qm[0];
-          const double vavg_n = 0.5 * ((qm[1] + qp[1]) * nrm[0] + (qm[2] + qp[2]) * nrm[1] +
-                                       (qm[3] + qp[3]) * nrm[2]);
-          const double vjump = (qp[1] - qm[1]) * nrm[0] + (qp[2] - qm[2]) * nrm[1] +
-                               (qp[3] - qm[3]) * nrm[2];
-          // u: Fscale * the flux's u part; v: s = c ({u} + [v].n / 2), with
-          // c_{f,r} of this lane's face applied per t_r below
-          const double bu = fok ? g23.y * (c0 * (vavg_n + 0.5 * udiff)) : 0.0;
-          const double bs = fok ? c0 * (uavg + 0.5 * vjump) : 0.0;
+          // the flux's u part times Fscale, and s of its velocity part s n
+          // (c_{f,r} of this lane's face applied per t_r below)
+          double bu, bs;
+          if constexpr (lean_flux<EQ, ORDER>()) {
+            // with a = v-.n, b = v+.n and h = c / 2 (wave_flux): u part
+            // h (a + b + u+ - u-), s = h (u- + u+ + b - a)
+            const double am = qm[1] * g01.x + qm[2] * g01.y + qm[3] * g23.x;
+            const double bp = qp[1] * g01.x + qp[2] * g01.y + qp[3] * g23.x;
+            const double h = 0.5 * c0;
+            bu = (h * g23.y) * ((am + bp) + (qp[0] - qm[0]));
+            bs = h * ((qm[0] + qp[0]) + (bp - am));
+          } else {
+            // wave_flux as written: c ({v}.n + [u]/2) and s = c ({u} + [v].n/2)
+            const double uavg = 0.5 * (qm[0] + qp[0]);
+            const double udiff = qp[0] - qm[0];
+            const double vavg_n = 0.5 * ((qm[1] + qp[1]) * g01.x + (qm[2] + qp[2]) * g01.y +
+                                         (qm[3] + qp[3]) * g23.x);
+            const double vjump = (qp[1] - qm[1]) * g01.x + (qp[2] - qm[2]) * g01.y +
+                                 (qp[3] - qm[3]) * g23.x;
+            bu = g23.y * (c0 * (vavg_n + 0.5 * udiff));
+            bs = c0 * (uavg + 0.5 * vjump);
+          }
+          bu = fok ? bu : 0.0;
+          bs = fok ? bs : 0.0;
           // faces this k-step covers, over its four lane groups
           constexpr int FPK = (NFP + 3) / 4;
           const int f_lo = direct_pair_of(EQ, ORDER, ks, 0) >= 0
@@ -671,7 +716,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
           double qm[NF], qp[NF], out[NF];
           if constexpr (ST) {
 #pragma unroll
-            for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * NP + fe * NP + s_fmask[fn]];
+            for (int c = 0; c < NF; ++c) qm[c] = stg[c * 16 * SP + fe * SP + s_fmask[fn]];
           } else {
 #pragma unroll
             for (int c = 0; c < NF; ++c) qm[c] = a.q.p[c][vn];
@@ -682,10 +727,16 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
             const int64_t src = ext >= 0 ? (int64_t)ext : vn;
 #pragma unroll
             for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][src];
-            double qb[NF];
-            Flux::boundary(qm, nrm, qb);
+            if constexpr (lean_flux<EQ, ORDER>()) {
 #pragma unroll
-            for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+              for (int c = 0; c < NF; ++c)
+                if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
+            } else {
+              double qb[NF];
+              Flux::boundary(qm, nrm, qb);
+#pragma unroll
+              for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+            }
           } else if (ext >= 0) {
 #pragma unroll
             for (int c = 0; c < NF; ++c) qp[c] = a.q.p[c][ext];
@@ -749,7 +800,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = 16 * b + g + 4 * v;
-          if (i < NP) buf[n * NP + i] = acc[c][b][v];
+          if (i < NP) buf[n * SP + i] = acc[c][b][v];
         }
       wave_lds_fence();
       if constexpr (POST) {
@@ -761,7 +812,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
           const int j = g * KS + ks;
-          const double bv = j < NP ? buf[n * NP + j] : 0.0;
+          const double bv = j < NP ? buf[n * SP + j] : 0.0;
 #pragma unroll
           for (int b = 0; b < MB; ++b)
             mma(pacc[b], s_post[(b * KS + ks) * 64 + lane], bv, SH && b == MB - 1);
@@ -773,7 +824,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int i = 16 * b + g + 4 * v;
-            if (i < NP) buf[n * NP + i] = sc * pacc[b][v];
+            if (i < NP) buf[n * SP + i] = sc * pacc[b][v];
           }
         wave_lds_fence();
       }
@@ -785,7 +836,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
         for (int t = 0; t < NSLOT; ++t) {
           const int idx = lane + 64 * t;
           if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem) {
-            const double r = (a.stage_a != 0.0 ? a.stage_a * st_res[t] : 0.0) + a.stage_dt * buf[idx];
+            const double r = (a.stage_a != 0.0 ? a.stage_a * st_res[t] : 0.0) + a.stage_dt * buf[(idx / NP) * SP + idx % NP];
             res[idx] = r;
             qo[idx] = a.stage_beta * st_q[t] + a.stage_b * r;
           }
@@ -793,7 +844,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
       } else {
         double* dst = a.rhs.p[c] + tile * 16 * NP;
         for (int idx = lane; idx < 16 * NP; idx += 64)
-          if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+          if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[(idx / NP) * SP + idx % NP], dst + idx);
       }
       wave_lds_fence();
     }
