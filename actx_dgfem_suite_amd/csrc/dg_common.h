@@ -70,6 +70,14 @@ struct RhsArgs {
   double p0;
 };
 
+// The schedule's unit at a wave-uniform position, loaded through the scalar
+// cache.  A vector load would be counted together with the wave's earlier
+// output stores, so waiting for it would wait for those stores too.
+__device__ inline int32_t sched_at(const int32_t* order, int32_t pos) {
+  typedef const __attribute__((address_space(4))) int32_t* const_i32;
+  return ((const_i32)order)[__builtin_amdgcn_readfirstlane(pos)];
+}
+
 // This block's XCD and its index among the blocks of that XCD.  Needs
 // gridDim.x % 8 == 0.
 __device__ inline int block_xcd() { return blockIdx.x & 7; }

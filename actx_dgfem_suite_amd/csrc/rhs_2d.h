@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void rhs_2d_kernel(Rhs2DArgs<T> a) {
   }
 
   for (int32_t pos = a.sched.part[block_xcd()] + block_in_xcd(); pos < p_end; pos += p_step) {
-    const int64_t b = a.sched.order[pos];
+    const int64_t b = sched_at(a.sched.order, pos);
     const int64_t e0 = b * EB;
     const int nel = (int)(a.nelem - e0 < EB ? a.nelem - e0 : EB);
     // exterior indices and values of this batch's face nodes, issued before

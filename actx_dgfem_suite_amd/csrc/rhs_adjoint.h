@@ -524,11 +524,11 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
 
   int32_t pos = a.sched.part[block_xcd()] + block_in_xcd();
   if (pos < p_end) {
-    issue_map(a.sched.order[pos]);
+    issue_map(sched_at(a.sched.order, pos));
     issue_gathers();
   }
   for (; pos < p_end; pos += p_step) {
-    const int64_t b = a.sched.order[pos];
+    const int64_t b = sched_at(a.sched.order, pos);
     const int64_t e0 = b * EB;
     const int nel = batch_nel(b);
     __syncthreads();
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void rhs_vjp_kernel(VjpArgs<T> a) {
       }
     }
     const bool has_next = pos + p_step < p_end;
-    if (has_next) issue_map(a.sched.order[pos + p_step]);
+    if (has_next) issue_map(sched_at(a.sched.order, pos + p_step));
     // The staging loads of a round are issued together before their LDS
     // stores (a load-then-store loop per field would wait out one memory
     // round trip per field): mu and Fscale, then the geometry, then q

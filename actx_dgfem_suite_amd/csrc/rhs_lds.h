@@ -83,11 +83,11 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
 
   int32_t pos = a.sched.part[block_xcd()] + block_in_xcd();
   if (pos < p_end) {
-    issue_map(a.sched.order[pos]);
+    issue_map(sched_at(a.sched.order, pos));
     issue_gathers();
   }
   for (; pos < p_end; pos += p_step) {
-    const int64_t b = a.sched.order[pos];
+    const int64_t b = sched_at(a.sched.order, pos);
     const int64_t e0 = b * EB;
     const int nel = batch_nel(b);
     const bool has_next = pos + p_step < p_end;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       for (int c = 0; c < NF; ++c) s_fl[c][fe * NFN + fn] = g[3] * out[c];
     }
     if (has_next) {
-      issue_map(a.sched.order[pos + p_step]);
+      issue_map(sched_at(a.sched.order, pos + p_step));
       issue_gathers();
     }
     __syncthreads();

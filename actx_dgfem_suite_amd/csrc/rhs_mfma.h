@@ -378,13 +378,10 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
       ext_pf[it] = a.ext_t[t * flux_items_of(EQ, ORDER) + (ch * C + it) * 64 + lane];
   };
 
-  // The next tile's index is loaded while this tile runs, through the scalar
-  // cache (the wave's position is uniform): a vector load would be counted
-  // with the tile's output stores, and waiting for it would wait for them.
-  typedef const __attribute__((address_space(4))) int32_t* const_i32;
-  const const_i32 order = (const_i32)a.sched.order;
+  // the next tile's index is loaded while this tile runs (sched_at: through
+  // the scalar cache)
   int32_t pos = __builtin_amdgcn_readfirstlane(a.sched.part[block_xcd()] + slot);
-  int32_t next_tile = pos < p_end ? order[pos] : 0;
+  int32_t next_tile = pos < p_end ? sched_at(a.sched.order, pos) : 0;
 
   // volume operands of a tile: state at the lane's B-operand nodes
   // j = g * KS + ks of element n (0 past Np), and rx
@@ -407,7 +404,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 
   for (; pos < p_end; pos += nslots) {
     const int64_t tile = next_tile;
-    if (pos + nslots < p_end) next_tile = order[pos + nslots];
+    if (pos + nslots < p_end) next_tile = sched_at(a.sched.order, pos + nslots);
     // direct flux: every k-step's exterior-map entry, loaded with the volume
     // operands so that no gather waits on the map (wave: -1.5%; it costs
     // registers that Maxwell P4 cannot spare, +9%)

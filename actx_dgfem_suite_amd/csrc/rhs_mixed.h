@@ -331,8 +331,13 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     return &s_lift[(((b * (KSL / KG)) + kq) * 64 + lane) * AW];
   };
 
-  for (int32_t pos = a.sched.part[block_xcd()] + slot; pos < p_end; pos += nslots) {
-    const int64_t tile = a.sched.order[pos];
+  // the next tile's index is loaded while this tile runs (sched_at: through
+  // the scalar cache)
+  int32_t pos = __builtin_amdgcn_readfirstlane(a.sched.part[block_xcd()] + slot);
+  int32_t next_tile = pos < p_end ? sched_at(a.sched.order, pos) : 0;
+  for (; pos < p_end; pos += nslots) {
+    const int64_t tile = next_tile;
+    if (pos + nslots < p_end) next_tile = sched_at(a.sched.order, pos + nslots);
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
@@ -698,9 +703,20 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
           }
         }
       } else {
+        // all of the lane's slots read from LDS, then stored
         float* dst = a.rhs[c] + tile * 16 * NP;
-        for (int idx = lane; idx < 16 * NP; idx += 64)
-          if (tile * 16 + idx / NP < a.nelem) __builtin_nontemporal_store(buf[idx], dst + idx);
+        float vals[NSLOT];
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          vals[t] = idx < 16 * NP ? buf[idx] : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < NSLOT; ++t) {
+          const int idx = lane + 64 * t;
+          if (idx < 16 * NP && tile * 16 + idx / NP < a.nelem)
+            __builtin_nontemporal_store(vals[t], dst + idx);
+        }
       }
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     }
