@@ -1186,6 +1186,16 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     p->kernel = {nullptr, kBlock, kBlock / np};
   } else {
     p->variant = d->variant == DG_KERNEL_AUTO ? auto_variant(d->equation, d->order) : d->variant;
+    if (p->variant == DG_KERNEL_MFMA && intile_neighbours_of(d->equation, d->order) &&
+        d->nelements * np * 8 >= ((int64_t)1 << 32)) {
+      if (d->variant == DG_KERNEL_MFMA) {
+        free_plan(p);
+        return fail(DG_ERR_UNSUPPORTED,
+                    "MFMA kernel: each state field must stay below 4 GiB at this order (32-bit "
+                    "buffer offsets); the LDS-staged kernel takes larger meshes");
+      }
+      p->variant = DG_KERNEL_LDS;
+    }
     if (p->variant == DG_KERNEL_MFMA && ref_normal_lift_of(d->equation, d->order) &&
         !geometry_fits_ref_normals(d->nelements, d->geometry)) {
       if (d->variant == DG_KERNEL_MFMA) {
@@ -1244,6 +1254,19 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     if (direct_flux_of(d->equation, d->order)) {
       std::vector<int32_t> ext_d;
       build_direct_map(k, d->equation, d->order, d->ext_map, ext_d);
+      if (intile_neighbours_of(d->equation, d->order)) {
+        // neighbours inside the tile point into the staged state
+        const int ksd = direct_ksteps_of(d->equation, d->order);
+        const int64_t nt = (k + 15) / 16;
+        for (int64_t t = 0; t < nt; ++t)
+          for (int ks = 0; ks < ksd; ++ks)
+            for (int l = 0; l < 64; ++l) {
+              int32_t& v = ext_d[((size_t)t * ksd + ks) * 64 + l];
+              const int64_t e = v / np;
+              if (v >= 0 && e / 16 == t && direct_pair_of(d->equation, d->order, ks, l >> 4) >= 0)
+                v = kInTile | (int32_t)((e % 16) << 8) | (v % np);
+            }
+      }
       if ((rc = upload(&p->ext_d, ext_d.data(), ext_d.size()))) {
         free_plan(p);
         return rc;
