@@ -136,10 +136,11 @@ __host__ __device__ constexpr int flux_items_of(int eq, int order) {
 // reference normals (ref_normal_lift).  Measured on MI355X against the
 // chunked producer: Maxwell P2 79 against 88 us for the LDS-staged kernel,
 // Maxwell P4 -3%; with the reference-normal LIFT, wave P2 -1.5%, P4 -2.3%;
-// slower elsewhere (Euler +20%, where the padded face nodes cost a third more
-// flux evaluations; wave P1 +15%).
+// Euler P4 (packed k-steps) -6%; slower elsewhere (Euler P2 +17%, P3 +22%,
+// Maxwell P3 +26%, wave P1 +15%).
 __host__ __device__ constexpr bool direct_flux_of(int eq, int order) {
-  return (eq == DG_EQ_WAVE && order >= 2) || (eq == DG_EQ_MAXWELL && (order == 2 || order == 4));
+  return (eq == DG_EQ_WAVE && order >= 2) || (eq == DG_EQ_MAXWELL && (order == 2 || order == 4)) ||
+         (eq == DG_EQ_EULER && order == 4);
 }
 template <int EQ, int ORDER>
 constexpr bool direct_flux() { return direct_flux_of(EQ, ORDER); }
