@@ -70,6 +70,13 @@ struct RhsArgs {
   double p0;
 };
 
+// Fourth word of a raw buffer resource on gfx950 (32-bit data format).
+constexpr int kBufferWord3 = 0x00020000;
+
+// Exterior-map entries with this bit set point into a kernel's LDS copy of the
+// state of its current unit (tile or batch) instead of into the state arrays.
+constexpr int32_t kInTile = 1 << 30;
+
 // The schedule's unit at a wave-uniform position, loaded through the scalar
 // cache.  A vector load would be counted together with the wave's earlier
 // output stores, so waiting for it would wait for those stores too.

@@ -793,6 +793,7 @@ struct dg_plan {
   int variant = 0;
   double* geo = nullptr;
   int32_t* emap = nullptr;
+  int32_t* emap_b = nullptr;   // adjoint kernel: in-batch neighbours encoded (kInTile)
   double* dmat_t = nullptr;
   double* lift_t = nullptr;
   int32_t* fmask = nullptr;
@@ -845,6 +846,7 @@ void free_plan(dg_plan* p) {
   if (!p) return;
   (void)hipFree(p->geo);
   (void)hipFree(p->emap);
+  (void)hipFree(p->emap_b);
   (void)hipFree(p->dmat_t);
   (void)hipFree(p->lift_t);
   (void)hipFree(p->fmask);
@@ -1380,6 +1382,22 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     }
     p->vjp_kernel = pick_vjp<double>(d->equation, d->order, p->dim);
     p->vjp_kernel32 = pick_vjp<float>(d->equation, d->order, p->dim);
+    {
+      // the adjoint kernel's exterior map: neighbours inside the element's
+      // batch of kBlock / Np elements point into the batch's staged state
+      const int eb = kBlock / np;
+      std::vector<int32_t> mb(d->ext_map, d->ext_map + (size_t)k * nfn);
+      for (int64_t e = 0; e < k; ++e)
+        for (int t = 0; t < nfn; ++t) {
+          int32_t& v = mb[(size_t)e * nfn + t];
+          if (v >= 0 && v / np / eb == e / eb)
+            v = kInTile | (int32_t)(((v / np) % eb) << 8) | (v % np);
+        }
+      if ((rc = upload(&p->emap_b, mb.data(), mb.size()))) {
+        free_plan(p);
+        return rc;
+      }
+    }
     p->mass_kernel = pick_mass<double>(np, p->nfields);
     p->mass_kernel32 = pick_mass<float>(np, p->nfields);
     const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
@@ -1665,6 +1683,10 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
     }
   }
 
+  // the adjoint kernel addresses the fields with 32-bit buffer offsets
+  if (fbytes >= ((size_t)1 << 32))
+    return fail(DG_ERR_UNSUPPORTED, std::string(name) +
+                                        ": each field must stay below 4 GiB for the adjoint kernel");
   VjpArgs<T> a{};
   for (int c = 0; c < p->nfields; ++c) {
     a.q[c] = q ? q[c] : nullptr;
@@ -1681,6 +1703,7 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
   a.sched = {p->sched_batch, p->part_batch};
   a.geo = p->geo;
   a.emap = p->emap;
+  a.emap_b = p->emap_b;
   a.fmask = p->fmask;
   a.amat_t = p->amat_t;
   a.face_mass = p->face_mass;
