@@ -569,8 +569,9 @@ std::string build_adjoint_data(const dg_plan_desc* d, int np, int nfp, int nface
 
 // A tiles of the mixed kernel (layout in rhs_mixed.h): 32-bit words, either
 // one float per lane (FP32) or four (hi, lo) bf16 pairs packed in 4 words.
-void build_mixed_tiles(int prec, int kgb, int np, int nfp, const double* op, const double* lift,
+void build_mixed_tiles(int prec, int kgb, int eq, int order, const double* op, const double* lift,
                        std::vector<uint32_t>& op_words, std::vector<uint32_t>& lift_words) {
+  const int np = np_of(order), nfp = nfp_of(order);
   const int kg = prec == DG_PREC_BF16X3 ? kgb : 1;
   const int aw = kg;
   const int mb = (np + 15) / 16;
@@ -611,6 +612,20 @@ void build_mixed_tiles(int prec, int kgb, int np, int nfp, const double* op, con
           }
           pack(op_words, ((((size_t)r * mb + b) * (ks_n / kg) + kq) * 64 + l) * aw, vals);
         }
+  if (mixed_direct_flux_of(eq, prec, order)) {
+    // the direct flux's k order (FP32 only): lane group g of k-step ks holds
+    // face-node pair direct_pair_of(eq, order, ks, g)
+    const int ksd = direct_ksteps_of(eq, order);
+    lift_words.assign((size_t)mb * ksd * 64, 0u);
+    for (int b = 0; b < mb; ++b)
+      for (int ks = 0; ks < ksd; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int m = 16 * b + (l & 15), fp = direct_pair_of(eq, order, ks, l >> 4);
+          vals[0] = (m < np && fp >= 0) ? lift[(size_t)m * nfn + fp] : 0.0;
+          pack(lift_words, ((size_t)b * ksd + ks) * 64 + l, vals);
+        }
+    return;
+  }
   for (int b = 0; b < mb; ++b)
     for (int kq = 0; kq < ksl / kg; ++kq)
       for (int l = 0; l < 64; ++l) {
@@ -969,6 +984,7 @@ MixedArgs mixed_args(const dg_plan* p, int k, const float* const* in, float* con
   a.vgeo = p->vgeo;
   a.fgeo = p->fgeo;
   a.ext_t = p->ext_t;
+  a.ext_d = p->ext_d;
   a.op_tiles = p->mixed_op[k];
   a.lift_tiles = p->mixed_lift[k];
   a.nelem = p->nelem;
@@ -1331,7 +1347,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
   for (int k = 0; k < 2 && !tri; ++k) {
     const int prec = k == 0 ? DG_PREC_FP32 : DG_PREC_BF16X3;
     std::vector<uint32_t> ow, lw;
-    build_mixed_tiles(prec, bf16_kgroup_of(d->equation, d->order), np, nfp, dsrc, d->lift, ow, lw);
+    build_mixed_tiles(prec, bf16_kgroup_of(d->equation, d->order), d->equation, d->order, dsrc,
+                      d->lift, ow, lw);
     if ((rc = upload(&p->mixed_op[k], ow.data(), ow.size())) ||
         (rc = upload(&p->mixed_lift[k], lw.data(), lw.size()))) {
       free_plan(p);

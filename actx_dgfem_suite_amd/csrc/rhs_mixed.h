@@ -51,6 +51,7 @@ struct MixedArgs {
   const double* vgeo;       // [tile][9][16]
   const double* fgeo;       // [tile][face*16 + n][4]
   const int32_t* ext_t;     // [tile][chunk][item][64]
+  const int32_t* ext_d;     // [tile][kstep][64]: the direct flux order (in-tile entries marked)
   const void* op_tiles;     // A tiles of D (float, or bf16 hi/lo pairs)
   const void* lift_tiles;   // A tiles of LIFT
   int64_t nelem;
@@ -279,6 +280,14 @@ constexpr bool mixed_maxwell_bform() {
   return PREC == DG_PREC_FP32 || ORDER == 3;
 }
 
+// Whether the FP32 wave kernel takes the FP64 kernel's direct flux with the
+// velocity flux LIFTed in reference normals and in-tile neighbours from the
+// staged state (rhs_mfma.h): its LIFT tiles in the direct k order, the
+// exterior map from ext_d.
+__host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int order) {
+  return eq == DG_EQ_WAVE && prec == DG_PREC_FP32 && direct_flux_of(eq, order);
+}
+
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
 // res = a res + dt rhs, q = q + b res, q32 = (float)q to it in its output
 // phase (the FP32 RHS tile is the one the unfused path would store).
@@ -294,11 +303,21 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   constexpr int NF = nfields_of(EQ);
   constexpr int WAVES = kBlock / 64;
   constexpr int FROW = kMixedFluxRow;
+  // direct flux (FP32 wave): LIFT k-steps and tiles in the direct order, no
+  // flux buffer
+  constexpr bool DFM = mixed_direct_flux_of(EQ, PREC, ORDER);
+  constexpr int KSD = direct_ksteps_of(EQ, ORDER);
+  constexpr int LW = DFM ? MB * KSD * 64 : S::LIFT_WORDS;
   // flux buffer: KSL k-steps x NF fields x FROW; output: 16 x NP of one field
-  constexpr int WB = (KSL * NF * FROW > 16 * NP) ? KSL * NF * FROW : 16 * NP;
+  constexpr int WB = (!DFM && KSL * NF * FROW > 16 * NP) ? KSL * NF * FROW : 16 * NP;
 
   __shared__ uint32_t s_op[S::OP_WORDS];
-  __shared__ uint32_t s_lift[S::LIFT_WORDS];
+  __shared__ uint32_t s_lift[LW];
+  // direct flux at P3: the tile's face geometry [face * 16 + n][4] in LDS and
+  // every k-step's map entry loaded at the tile's start (measured: FP32 wave
+  // P3 -10%; at P2 +5% and at P4 even, where its LDS costs a block per CU)
+  constexpr bool DFG = DFM && ORDER == 3;
+  __shared__ double2 s_fgeo[kBlock / 64][DFG ? 128 : 1];
   __shared__ int s_fmask[NFN];
   __shared__ float s_buf[WAVES][WB];
   constexpr bool ST = mixed_stage_own_traces<EQ, PREC, ORDER>();
@@ -310,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     const uint32_t* op = (const uint32_t*)a.op_tiles;
     const uint32_t* lf = (const uint32_t*)a.lift_tiles;
     for (int k = tid; k < S::OP_WORDS; k += kBlock) s_op[k] = op[k];
-    for (int k = tid; k < S::LIFT_WORDS; k += kBlock) s_lift[k] = lf[k];
+    for (int k = tid; k < LW; k += kBlock) s_lift[k] = lf[k];
     for (int k = tid; k < NFN; k += kBlock) s_fmask[k] = a.fmask[k];
   }
   __syncthreads();
@@ -338,6 +357,18 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   for (; pos < p_end; pos += nslots) {
     const int64_t tile = next_tile;
     if (pos + nslots < p_end) next_tile = sched_at(a.sched.order, pos + nslots);
+    // direct flux: the tile's face geometry into LDS and every k-step's map
+    // entry, with the volume operands
+    double2* const fgl = s_fgeo[tid >> 6];
+    int32_t ext_all[DFG ? KSD : 1];
+    if constexpr (DFG) {
+      const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
+      const double2 f0 = src[lane], f1 = src[lane + 64];
+      fgl[lane] = f0;
+      fgl[lane + 64] = f1;
+#pragma unroll
+      for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
+    }
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
     const int64_t el = valid ? gel : a.nelem - 1;   // clamped for loads
@@ -499,9 +530,16 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         for (int v = 0; v < 4; ++v) {
           const double t0 = acc[1][b][v], t1 = acc[2][b][v], t2 = acc[3][b][v];
           acc[0][b][v] = (float)(-c0 * (double)acc[0][b][v]);
+          if constexpr (DFM) {
+            // t_r = -c D_r u stay, to take the velocity flux (combined below)
+            acc[1][b][v] = (float)(-c0 * t0);
+            acc[2][b][v] = (float)(-c0 * t1);
+            acc[3][b][v] = (float)(-c0 * t2);
+          } else {
 #pragma unroll
-          for (int x = 0; x < 3; ++x)
-            acc[1 + x][b][v] = (float)(-c0 * (rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2));
+            for (int x = 0; x < 3; ++x)
+              acc[1 + x][b][v] = (float)(-c0 * (rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2));
+          }
         }
     }
     if constexpr (EQ == DG_EQ_MAXWELL && !MXB) {
@@ -562,6 +600,91 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     // {{{ surface terms: flux in FP64, staged as FP32 B operands
 
     if constexpr (ST) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // staged state complete
+
+    if constexpr (DFM) {
+      // FP32 wave, the FP64 kernel's direct flux: at k-step ks lane (g, n)
+      // evaluates the upwind flux at face-node pair direct_pair_of(ks, g) of
+      // element n; the u part goes to acc[0], the velocity part s n to the
+      // t_r in reference normals (c_{f,r}); in-tile neighbours from the
+      // staged state, the others by buffer loads whose out-of-range lanes
+      // touch no memory
+      static_assert(ST, "the direct flux reads the staged tile");
+      const uint32_t field_bytes = (uint32_t)(a.nelem * NP * 4);
+      __amdgpu_buffer_rsrc_t rsq[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c)
+        rsq[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.q[c], (short)0, field_bytes, kBufferWord3);
+#pragma unroll
+      for (int ks = 0; ks < KSD; ++ks) {
+        const int fp = direct_pair_of(EQ, ORDER, ks, g);
+        const bool fok = fp >= 0;
+        const int f = fok ? fp / NFP : ks / ((NFP + 3) / 4);
+        const int vloc = s_fmask[fok ? fp : f * NFP];
+        double2 g01, g23;
+        int32_t ext;
+        if constexpr (DFG) {
+          g01 = fgl[2 * (f * 16 + n)];
+          g23 = fgl[2 * (f * 16 + n) + 1];
+          ext = ext_all[ks];
+        } else {
+          const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + f * 16 + n) * 4);
+          g01 = pgeo[0];
+          g23 = pgeo[1];
+          ext = a.ext_d[(tile * KSD + ks) * 64 + lane];
+        }
+        const bool loc = ext >= kInTile;
+        const bool glob = ext >= 0 && !loc;
+        const int lofs = loc ? ((ext >> 8) & 15) * NP + (ext & 255) : n * NP + vloc;
+        const uint32_t boff = glob ? (uint32_t)ext * 4u : field_bytes;
+        const float wl = glob ? 0.f : 1.f;
+        double qm[4], qp[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          qm[c] = stg[c * 16 * NP + n * NP + vloc];
+          const float gv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsq[c], boff, 0, 0));
+          qp[c] = (double)__builtin_fmaf(stg[c * 16 * NP + lofs], wl, gv);
+        }
+        qp[0] = ext >= 0 ? qp[0] : -qp[0];     // mirror boundary: u+ = -u-, v+ = v-
+        const double am = qm[1] * g01.x + qm[2] * g01.y + qm[3] * g23.x;
+        const double bp = qp[1] * g01.x + qp[2] * g01.y + qp[3] * g23.x;
+        const double h = 0.5 * c0;
+        float bu[1], bs[1];
+        bu[0] = fok ? (float)((h * g23.y) * ((am + bp) + (qp[0] - qm[0]))) : 0.f;
+        const double sv = fok ? h * ((qm[0] + qp[0]) + (bp - am)) : 0.0;
+        // faces this k-step covers, over its four lane groups
+        const int f_lo = direct_pair_of(EQ, ORDER, ks, 0) >= 0 ? direct_pair_of(EQ, ORDER, ks, 0) / NFP
+                                                                : ks / ((NFP + 3) / 4);
+        int f_hi = f_lo;
+#pragma unroll
+        for (int gg = 1; gg < 4; ++gg)
+          if (direct_pair_of(EQ, ORDER, ks, gg) >= 0) f_hi = direct_pair_of(EQ, ORDER, ks, gg) / NFP;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+          const uint32_t* av = &s_lift[(b * KSD + ks) * 64 + lane];
+          acc[0][b] = M::mma(av, bu, acc[0][b]);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            bool any = false;
+#pragma unroll
+            for (int ff = 0; ff < 4; ++ff) any |= ff >= f_lo && ff <= f_hi && ref_face_coef(ff, r) != 0;
+            if (any) {
+              bs[0] = (float)((double)ref_face_coef(f, r) * sv);
+              acc[1 + r][b] = M::mma(av, bs, acc[1 + r][b]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double t0 = acc[1][b][v], t1 = acc[2][b][v], t2 = acc[3][b][v];
+#pragma unroll
+          for (int x = 0; x < 3; ++x)
+            acc[1 + x][b][v] = (float)(rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2);
+        }
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // reads of the staged tile done
+    } else {
 
     // With the tile tables the items run in the FP64 MFMA kernel's chunk
     // order (C LIFT k-steps per chunk; C divides Nfp), so the face geometry
@@ -655,6 +778,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
       }
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
 
     // }}}
 
