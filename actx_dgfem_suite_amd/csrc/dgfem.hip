@@ -613,16 +613,19 @@ void build_mixed_tiles(int prec, int kgb, int eq, int order, const double* op, c
           pack(op_words, ((((size_t)r * mb + b) * (ks_n / kg) + kq) * 64 + l) * aw, vals);
         }
   if (mixed_direct_flux_of(eq, prec, order)) {
-    // the direct flux's k order (FP32 only): lane group g of k-step ks holds
-    // face-node pair direct_pair_of(eq, order, ks, g)
-    const int ksd = direct_ksteps_of(eq, order);
-    lift_words.assign((size_t)mb * ksd * 64, 0u);
+    // the direct flux's k order: k index i of lane group g in instruction kq
+    // holds face-node pair mixed_pair_of(eq, prec, order, kq, g, i)
+    const int kqd = mixed_direct_kq_of(eq, prec, order);
+    lift_words.assign((size_t)mb * kqd * 64 * aw, 0u);
     for (int b = 0; b < mb; ++b)
-      for (int ks = 0; ks < ksd; ++ks)
+      for (int kq = 0; kq < kqd; ++kq)
         for (int l = 0; l < 64; ++l) {
-          const int m = 16 * b + (l & 15), fp = direct_pair_of(eq, order, ks, l >> 4);
-          vals[0] = (m < np && fp >= 0) ? lift[(size_t)m * nfn + fp] : 0.0;
-          pack(lift_words, ((size_t)b * ksd + ks) * 64 + l, vals);
+          const int m = 16 * b + (l & 15);
+          for (int i = 0; i < kg; ++i) {
+            const int fp = mixed_pair_of(eq, prec, order, kq, l >> 4, i);
+            vals[i] = (m < np && fp >= 0) ? lift[(size_t)m * nfn + fp] : 0.0;
+          }
+          pack(lift_words, (((size_t)b * kqd + kq) * 64 + l) * aw, vals);
         }
     return;
   }

@@ -284,8 +284,26 @@ constexpr bool mixed_maxwell_bform() {
 // velocity flux LIFTed in reference normals and in-tile neighbours from the
 // staged state (rhs_mfma.h): its LIFT tiles in the direct k order, the
 // exterior map from ext_d.
+//
+// BF16x3 (four k per lane per instruction) takes it at P3, where the FP64
+// kernel's k order is packed: instruction kq, lane group g and its i-th k hold
+// face-node pair 16 kq + 4 g + i, which is slot (4 kq + g, lane 16 i + n) of
+// the FP64 kernel's exterior map ext_d.
 __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int order) {
-  return eq == DG_EQ_WAVE && prec == DG_PREC_FP32 && direct_flux_of(eq, order);
+  // BF16x3 P2 measured 30% slower on the direct flux (its registers cost a
+  // wave per SIMD); P3 10% faster
+  return eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
+         (prec == DG_PREC_FP32 ||
+          (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4));
+}
+// Instructions of the direct LIFT contraction and the face-node pair of k
+// index i of lane group g in instruction kq (-1: padding).
+__host__ __device__ constexpr int mixed_direct_kq_of(int eq, int prec, int order) {
+  return prec == DG_PREC_FP32 ? direct_ksteps_of(eq, order) : (4 * nfp_of(order) + 15) / 16;
+}
+__host__ __device__ constexpr int mixed_pair_of(int eq, int prec, int order, int kq, int g, int i) {
+  return prec == DG_PREC_FP32 ? direct_pair_of(eq, order, kq, g)
+                              : (16 * kq + 4 * g + i < 4 * nfp_of(order) ? 16 * kq + 4 * g + i : -1);
 }
 
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
@@ -306,8 +324,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // direct flux (FP32 wave): LIFT k-steps and tiles in the direct order, no
   // flux buffer
   constexpr bool DFM = mixed_direct_flux_of(EQ, PREC, ORDER);
-  constexpr int KSD = direct_ksteps_of(EQ, ORDER);
-  constexpr int LW = DFM ? MB * KSD * 64 : S::LIFT_WORDS;
+  constexpr int KSD = direct_ksteps_of(EQ, ORDER);         // the FP64 map's k-steps
+  constexpr int KQD = mixed_direct_kq_of(EQ, PREC, ORDER); // direct LIFT instructions
+  constexpr int LW = DFM ? MB * KQD * 64 * AW : S::LIFT_WORDS;
   // flux buffer: KSL k-steps x NF fields x FROW; output: 16 x NP of one field
   constexpr int WB = (!DFM && KSL * NF * FROW > 16 * NP) ? KSL * NF * FROW : 16 * NP;
 
@@ -320,7 +339,8 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   __shared__ double2 s_fgeo[kBlock / 64][DFG ? 128 : 1];
   __shared__ int s_fmask[NFN];
   __shared__ float s_buf[WAVES][WB];
-  constexpr bool ST = mixed_stage_own_traces<EQ, PREC, ORDER>();
+  // (the direct flux has no flux buffer and always stages the tile)
+  constexpr bool ST = mixed_stage_own_traces<EQ, PREC, ORDER>() || DFM;
   __shared__ float s_stage[WAVES][ST ? NF * 16 * NP : 1];
   float* const stg = s_stage[threadIdx.x >> 6];
 
@@ -360,14 +380,16 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     // direct flux: the tile's face geometry into LDS and every k-step's map
     // entry, with the volume operands
     double2* const fgl = s_fgeo[tid >> 6];
-    int32_t ext_all[DFG ? KSD : 1];
+    constexpr bool MAPA = DFG && PREC == DG_PREC_FP32;   // map entries preloaded
+    int32_t ext_all[MAPA ? KSD : 1];
     if constexpr (DFG) {
       const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
       const double2 f0 = src[lane], f1 = src[lane + 64];
       fgl[lane] = f0;
       fgl[lane + 64] = f1;
+      if constexpr (MAPA)
 #pragma unroll
-      for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
+        for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
     }
     const int64_t gel = tile * 16 + n;
     const bool valid = gel < a.nelem;
@@ -614,6 +636,12 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
       for (int c = 0; c < NF; ++c)
         rsq[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.q[c], (short)0, field_bytes, kBufferWord3);
+      // Two forms of the same loop: per k-step with one k per lane (FP32 P2,
+      // P3), and per instruction over its KG k (BF16x3, and FP32 P4).  The
+      // compiler schedules them differently: measured on MI355X, the first
+      // is 24-39% faster at FP32 P2-P3 (the second serialises the buffer
+      // loads there), the second 14% faster at FP32 P4.
+      if constexpr (PREC == DG_PREC_FP32 && ORDER <= 3) {
 #pragma unroll
       for (int ks = 0; ks < KSD; ++ks) {
         const int fp = direct_pair_of(EQ, ORDER, ks, g);
@@ -625,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         if constexpr (DFG) {
           g01 = fgl[2 * (f * 16 + n)];
           g23 = fgl[2 * (f * 16 + n) + 1];
-          ext = ext_all[ks];
+          ext = ext_all[MAPA ? ks : 0];
         } else {
           const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + f * 16 + n) * 4);
           g01 = pgeo[0];
@@ -673,6 +701,90 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
             }
           }
         }
+      }
+      } else {
+#pragma unroll
+      for (int kq = 0; kq < KQD; ++kq) {
+        float bu[KG], sv[KG];
+        int fi[KG];
+#pragma unroll
+        for (int i = 0; i < KG; ++i) {
+          const int fp = mixed_pair_of(EQ, PREC, ORDER, kq, g, i);
+          const bool fok = fp >= 0;
+          const int f = fok ? fp / NFP : (PREC == DG_PREC_FP32 ? kq / ((NFP + 3) / 4) : 0);
+          fi[i] = f;
+          const int vloc = s_fmask[fok ? fp : f * NFP];
+          double2 g01, g23;
+          int32_t ext;
+          if constexpr (DFG) {
+            g01 = fgl[2 * (f * 16 + n)];
+            g23 = fgl[2 * (f * 16 + n) + 1];
+          } else {
+            const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + f * 16 + n) * 4);
+            g01 = pgeo[0];
+            g23 = pgeo[1];
+          }
+          if constexpr (PREC == DG_PREC_FP32) {
+            if constexpr (MAPA) ext = ext_all[kq];
+            else ext = a.ext_d[(tile * KSD + kq) * 64 + lane];
+          } else {
+            // pair fp sits at slot (fp / 4, lane 16 (fp % 4) + n) of ext_d;
+            // padding reads pair 0's (valid) and contributes nothing
+            const int pp = fok ? fp : 0;
+            ext = a.ext_d[(tile * KSD + pp / 4) * 64 + (pp % 4) * 16 + n];
+          }
+          const bool loc = ext >= kInTile;
+          const bool glob = ext >= 0 && !loc;
+          const int lofs = loc ? ((ext >> 8) & 15) * NP + (ext & 255) : n * NP + vloc;
+          const uint32_t boff = glob ? (uint32_t)ext * 4u : field_bytes;
+          const float wl = glob ? 0.f : 1.f;
+          double qm[4], qp[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            qm[c] = stg[c * 16 * NP + n * NP + vloc];
+            const float gv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsq[c], boff, 0, 0));
+            qp[c] = (double)__builtin_fmaf(stg[c * 16 * NP + lofs], wl, gv);
+          }
+          qp[0] = ext >= 0 ? qp[0] : -qp[0];     // mirror boundary: u+ = -u-, v+ = v-
+          const double am = qm[1] * g01.x + qm[2] * g01.y + qm[3] * g23.x;
+          const double bp = qp[1] * g01.x + qp[2] * g01.y + qp[3] * g23.x;
+          const double h = 0.5 * c0;
+          bu[i] = fok ? (float)((h * g23.y) * ((am + bp) + (qp[0] - qm[0]))) : 0.f;
+          sv[i] = fok ? (float)(h * ((qm[0] + qp[0]) + (bp - am))) : 0.f;
+        }
+        // faces this instruction covers, over its lane groups and k
+        int f_lo = 3, f_hi = 0;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+          for (int i = 0; i < KG; ++i) {
+            const int pq = mixed_pair_of(EQ, PREC, ORDER, kq, gg, i);
+            if (pq >= 0) {
+              f_lo = pq / NFP < f_lo ? pq / NFP : f_lo;
+              f_hi = pq / NFP > f_hi ? pq / NFP : f_hi;
+            }
+          }
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+          const uint32_t* av = &s_lift[((b * KQD + kq) * 64 + lane) * AW];
+          uint32_t aw[AW];
+#pragma unroll
+          for (int w = 0; w < AW; ++w) aw[w] = av[w];
+          acc[0][b] = M::mma(aw, bu, acc[0][b]);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            bool any = false;
+#pragma unroll
+            for (int ff = 0; ff < 4; ++ff) any |= ff >= f_lo && ff <= f_hi && ref_face_coef(ff, r) != 0;
+            if (any) {
+              float bs[KG];
+#pragma unroll
+              for (int i = 0; i < KG; ++i) bs[i] = (float)ref_face_coef(fi[i], r) * sv[i];
+              acc[1 + r][b] = M::mma(aw, bs, acc[1 + r][b]);
+            }
+          }
+        }
+      }
       }
 #pragma unroll
       for (int b = 0; b < MB; ++b)
