@@ -318,23 +318,19 @@ int auto_variant(int equation, int order) {
 
 // {{{ locality schedule
 
-// Spreads the bits of a 10-bit integer three apart (Morton interleave).
-uint32_t spread3(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
 // Work order for units of `unit` consecutive elements (see Schedule in
 // dg_common.h).  With element centroids, the units are split into 8 parts of
-// equal count by recursive coordinate bisection (longest axis at the median)
-// and each part is ordered along a Morton curve of the unit centroids: every
-// XCD works on a compact eighth of the mesh and sweeps it so that the units
-// in flight, and their face neighbours, stay close.  Without centroids the
-// units keep their storage order and the parts are 8 contiguous runs.
+// equal count by recursive coordinate bisection (longest axis at the median).
+// Each part is swept in kSlabsPerPart slabs across its longest axis; a slab
+// is walked in rows of kSlabRow units along the second-longest axis, and a
+// row along the third.  Every XCD thus works on a compact eighth of the mesh,
+// and the data a slab gathers from its predecessor was read a slab earlier:
+// at most a quarter of the part back, which the shared last-level cache
+// still holds when the state exceeds it.  Without centroids the units keep
+// their storage order and the parts are 8 contiguous runs.
+constexpr int kSlabsPerPart = 4;
+constexpr int kSlabRow = 16;
+
 void build_schedule(int64_t nelem, int unit, const double* centroids,
                     std::vector<int32_t>& order, std::array<int32_t, 9>& part) {
   const int64_t nunits = (nelem + unit - 1) / unit;
@@ -369,18 +365,22 @@ void build_schedule(int64_t nelem, int unit, const double* centroids,
     double lo[3], hi[3];
     bbox(r.b, r.e, lo, hi);
     if (r.np == 1) {
-      std::vector<std::pair<uint32_t, int32_t>> key(r.e - r.b);
-      for (int64_t k = r.b; k < r.e; ++k) {
-        uint32_t code = 0;
-        for (int d = 0; d < 3; ++d) {
-          const double w = hi[d] - lo[d];
-          const double t = w > 0 ? (c[order[k]][d] - lo[d]) / w : 0.0;
-          code |= spread3((uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0))) << d;
-        }
-        key[k - r.b] = {code, order[k]};
+      int ax[3] = {0, 1, 2};
+      std::stable_sort(ax, ax + 3, [&](int x, int y) { return hi[x] - lo[x] > hi[y] - lo[y]; });
+      auto along = [&](int axis) {
+        return [&c, axis](int32_t x, int32_t y) {
+          return c[x][axis] < c[y][axis] || (c[x][axis] == c[y][axis] && x < y);
+        };
+      };
+      const int64_t slab = std::max<int64_t>(1, (r.e - r.b + kSlabsPerPart - 1) / kSlabsPerPart);
+      std::sort(order.begin() + r.b, order.begin() + r.e, along(ax[0]));
+      for (int64_t sb = r.b; sb < r.e; sb += slab) {
+        const int64_t se = std::min<int64_t>(r.e, sb + slab);
+        std::sort(order.begin() + sb, order.begin() + se, along(ax[1]));
+        for (int64_t rb = sb; rb < se; rb += kSlabRow)
+          std::sort(order.begin() + rb, order.begin() + std::min<int64_t>(se, rb + kSlabRow),
+                    along(ax[2]));
       }
-      std::sort(key.begin(), key.end());
-      for (int64_t k = r.b; k < r.e; ++k) order[k] = key[k - r.b].second;
       continue;
     }
     int axis = 0;
