@@ -36,6 +36,7 @@
 #include "rhs_mfma.h"
 #include "rhs_mixed.h"
 #include "rhs_adjoint.h"
+#include "rhs_adjoint_mfma.h"
 
 // Workspace of dg_rhs_vjp_noalloc, for callers that cannot allocate: one
 // copy per GPU the library runs on, held until the process ends.  Host code
@@ -438,6 +439,80 @@ VjpKernel<T> pick_vjp_dim(int eq, int order) {
 template <typename T>
 VjpKernel<T> pick_vjp(int eq, int order, int dim) {
   return dim == 2 ? pick_vjp_dim<T, 2>(eq, order) : pick_vjp_dim<T, 3>(eq, order);
+}
+
+// The Euler adjoint on the matrix cores (FP64, tetrahedra).
+using VjpMfmaKernel = void (*)(VjpMfmaArgs);
+VjpMfmaKernel pick_vjp_mfma(int order) {
+  switch (order) {
+    case 1: return rhs_vjp_mfma_kernel<1>;
+    case 2: return rhs_vjp_mfma_kernel<2>;
+    case 3: return rhs_vjp_mfma_kernel<3>;
+    case 4: return rhs_vjp_mfma_kernel<4>;
+    default: return nullptr;
+  }
+}
+
+// A tiles of rhs_vjp_mfma_kernel (operand layouts in rhs_mfma.h; m(l) = l % 4
+// on an output block of at most 4 rows, else l % 16):
+//   volume: vol[(r * MB + b) * KS + ks][l] = A_r[16 b + m(l)][(l / 16) * KS + ks],
+//           A_r[i][j] = amat_t[r][j][i];
+//   faces:  face[f * KSF + ks][l] = Mf_f[b = l / 16 + 4 ks][a = m(l)] (the
+//           contraction psi[a] = sum_b Mf_f[b][a] mu[b]);
+//   own:    own[f * KS + ks][l] = P_f[a = m(l)][j = (l / 16) * KS + ks], P_f[a][j] =
+//           Mf_f[b][a] where fmask_f[b] = j: the same contraction on volume nodes.
+// The exterior map per tile, lane and face node: ext_v[t][l][f * KSF + v] =
+// ext_map[e][f * Nfp + a] for a = l / 16 + 4 v < Nfp (element e = 16 t + l % 16,
+// the last one repeated past the end), else -1.
+void build_vjp_mfma_tables(int64_t k, int np, int nfp, const double* amat_t, const double* fm,
+                           const int32_t* fmask, const int32_t* emap, std::vector<double>& vol,
+                           std::vector<double>& face, std::vector<double>& own,
+                           std::vector<int32_t>& ext_v) {
+  const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, ksf = (nfp + 3) / 4;
+  const bool shf = nfp <= 4;
+  vol.assign((size_t)3 * mb * ks_n * 64, 0.0);
+  for (int r = 0; r < 3; ++r)
+    for (int b = 0; b < mb; ++b)
+      for (int ks = 0; ks < ks_n; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int m = 16 * b + ((b == mb - 1 && short_last_block(np)) ? (l & 3) : (l & 15));
+          const int j = (l >> 4) * ks_n + ks;
+          if (m < np && j < np)
+            vol[(((size_t)r * mb + b) * ks_n + ks) * 64 + l] = amat_t[((size_t)r * np + j) * np + m];
+        }
+  face.assign((size_t)kFaces * ksf * 64, 0.0);
+  own.assign((size_t)kFaces * ks_n * 64, 0.0);
+  for (int f = 0; f < kFaces; ++f) {
+    for (int ks = 0; ks < ksf; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int aa = shf ? (l & 3) : (l & 15);
+        const int bb = (l >> 4) + 4 * ks;
+        if (aa < nfp && bb < nfp)
+          face[((size_t)f * ksf + ks) * 64 + l] = fm[((size_t)f * nfp + bb) * nfp + aa];
+      }
+    for (int ks = 0; ks < ks_n; ++ks)
+      for (int l = 0; l < 64; ++l) {
+        const int aa = shf ? (l & 3) : (l & 15);
+        const int j = (l >> 4) * ks_n + ks;
+        if (aa >= nfp || j >= np) continue;
+        for (int bb = 0; bb < nfp; ++bb)
+          if (fmask[f * nfp + bb] == j)
+            own[((size_t)f * ks_n + ks) * 64 + l] += fm[((size_t)f * nfp + bb) * nfp + aa];
+      }
+  }
+  const int64_t ntiles = (k + 15) / 16;
+  const int next = kFaces * ksf, nfn = kFaces * nfp;
+  ext_v.assign((size_t)ntiles * 64 * next, -1);
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int l = 0; l < 64; ++l) {
+      const int64_t e = std::min<int64_t>(t * 16 + (l & 15), k - 1);
+      for (int f = 0; f < kFaces; ++f)
+        for (int v = 0; v < ksf; ++v) {
+          const int aa = (l >> 4) + 4 * v;
+          if (aa < nfp)
+            ext_v[((size_t)t * 64 + l) * next + f * ksf + v] = emap[e * nfn + f * nfp + aa];
+        }
+    }
 }
 
 // Mass-operator kernels by node count (tetrahedra: 4, 10, 20, 35; triangles:
@@ -856,6 +931,13 @@ struct dg_plan {
   void* scratch = nullptr;       // this device's `scratch` (dg_rhs_vjp_noalloc)
   double* ext_fscale = nullptr;  // (K, F)
   int32_t* node_faces = nullptr; // (Np, 3)
+  // FP64 Euler adjoint on tetrahedra: the MFMA kernel and its A tiles
+  VjpMfmaKernel vjp_mfma_kernel = nullptr;
+  int vjp_mfma_grid = 0;
+  double* vjp_vol_tiles = nullptr;
+  double* vjp_face_tiles = nullptr;
+  double* vjp_own_tiles = nullptr;
+  int32_t* vjp_ext = nullptr;
 };
 
 namespace {
@@ -890,6 +972,10 @@ void free_plan(dg_plan* p) {
   (void)hipFree(p->post_tiles);
   (void)hipFree(p->ext_fscale);
   (void)hipFree(p->node_faces);
+  (void)hipFree(p->vjp_vol_tiles);
+  (void)hipFree(p->vjp_face_tiles);
+  (void)hipFree(p->vjp_own_tiles);
+  (void)hipFree(p->vjp_ext);
   delete p;
 }
 
@@ -1418,6 +1504,29 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
         return rc;
       }
     }
+    if (!tri && d->equation == DG_EQ_EULER) {
+      std::vector<double> vt, ft, ot;
+      std::vector<int32_t> xv;
+      build_vjp_mfma_tables(k, np, nfp, amat_t.data(), fm.data(), d->face_nodes, d->ext_map, vt,
+                            ft, ot, xv);
+      if ((rc = upload(&p->vjp_vol_tiles, vt.data(), vt.size())) ||
+          (rc = upload(&p->vjp_face_tiles, ft.data(), ft.size())) ||
+          (rc = upload(&p->vjp_own_tiles, ot.data(), ot.size())) ||
+          (rc = upload(&p->vjp_ext, xv.data(), xv.size()))) {
+        free_plan(p);
+        return rc;
+      }
+      p->vjp_mfma_kernel = pick_vjp_mfma(d->order);
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)p->vjp_mfma_kernel, 256,
+                                                        0) != hipSuccess ||
+          occ < 1)
+        occ = 1;
+      int64_t vg = (int64_t)prop.multiProcessorCount * occ;
+      const int64_t tbatch = ((k + 15) / 16 + 3) / 4;
+      if (vg > tbatch) vg = tbatch;
+      p->vjp_mfma_grid = (int)((vg + 7) / 8 * 8);
+    }
     p->mass_kernel = pick_mass<double>(np, p->nfields);
     p->mass_kernel32 = pick_mass<float>(np, p->nfields);
     const int64_t vbatch = (k + kBlock / np - 1) / (kBlock / np);
@@ -1707,6 +1816,34 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
   if (fbytes >= ((size_t)1 << 32))
     return fail(DG_ERR_UNSUPPORTED, std::string(name) +
                                         ": each field must stay below 4 GiB for the adjoint kernel");
+  if constexpr (!f32) {
+    if (p->vjp_mfma_kernel) {
+      // Euler on tetrahedra: the mass pass, then the MFMA adjoint kernel
+      hipLaunchKernelGGL(mk, dim3(p->mass_grid), dim3(kBlock), 0, (hipStream_t)stream, m);
+      DG_HIP_CHECK(hipGetLastError());
+      VjpMfmaArgs v{};
+      for (int c = 0; c < p->nfields; ++c) {
+        v.q[c] = q[c];
+        v.mu[c] = workspace[c];
+        v.qbar[c] = q_bar[c];
+      }
+      v.sched = {p->sched_tile, p->part_tile};
+      v.vgeo = p->vgeo;
+      v.fgeo = p->fgeo;
+      v.ext_v = p->vjp_ext;
+      v.fmask = p->fmask;
+      v.ext_fscale = p->ext_fscale;
+      v.vol_tiles = p->vjp_vol_tiles;
+      v.face_tiles = p->vjp_face_tiles;
+      v.own_tiles = p->vjp_own_tiles;
+      v.nelem = p->nelem;
+      v.p0 = p->params[0];
+      hipLaunchKernelGGL(p->vjp_mfma_kernel, dim3(p->vjp_mfma_grid), dim3(256), 0,
+                         (hipStream_t)stream, v);
+      DG_HIP_CHECK(hipGetLastError());
+      return DG_OK;
+    }
+  }
   VjpArgs<T> a{};
   for (int c = 0; c < p->nfields; ++c) {
     a.q[c] = q ? q[c] : nullptr;
