@@ -240,7 +240,7 @@ def main():
         vjp_ms = va.elapsed_time(vb) / args.adjoint_steps
         adjoint = {"vjp_us": vjp_ms * 1e3, "fwd_us": kernel_ms * 1e3,
                    "ratio": vjp_ms / kernel_ms, "steps": args.adjoint_steps,
-                   "kernels": ("mass_apply_kernel + rhs_vjp_kernel" if w.equation == "euler"
+                   "kernels": ("mass_apply_kernel + rhs_vjp_mfma_kernel" if w.equation == "euler"
                                else "mass_apply_kernel (R M^-1) + rhs_mfma_kernel with R M fused"
                                if plan.kernel_variant == "mfma"
                                else "mass_apply_kernel (R M^-1) + rhs_lds_kernel "
