@@ -4,7 +4,8 @@
   Maxwell P4 (25^3 x 6 = 93,750 tets): the FP64 RHS of the seeded benchmark
   state against the oracle on the whole mesh, per field;
 * Euler P2 at 40^3: the mixed forms (FP32 and BF16x3 contractions on an FP32
-  state) against the FP64 form, per field;
+  state) against the FP64 form, per field, and the FP64 VJP against the
+  FP32-storage VJP (two different adjoint kernels);
 * wave P4 at 46^3 x 6 = 584,016 tets (20.4M nodes per field): polynomial
   exactness and linearity of the FP64 RHS, and the gradient of the
   final-time misfit after 200 LSERK4 steps (checkpoints every 20) with a
@@ -73,6 +74,27 @@ def test_full_size_fp64_matches_oracle(name, device):
         errs = field_errors(got, ref)
     print(f"  rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
     assert max(errs) < TOL_FP64, errs
+
+
+def test_euler_p2_vjp_full_size(device):
+    """The FP64 Euler VJP (the MFMA adjoint kernel on 16-element tiles) on
+    the whole 40^3 mesh against the FP32-storage VJP, which runs the batch
+    adjoint kernel (one thread per node, FP64 arithmetic): independent code
+    on the same FP32-representable state and cotangent.  They differ by the
+    FP32 rounding of the latter's workspace and result (measured 4.6e-8)."""
+    w, d, q = _workload("euler_p2")
+    print(f"[case] full-size euler_p2 vjp K={d.nelements}", flush=True)
+    plan = DGPlan(d, w.equation, w.param, device=device)
+    q32 = _dev(q, device, torch.float32)
+    y32 = _dev(np.random.default_rng(7).standard_normal((5, d.nelements, d.np)), device,
+               torch.float32)
+    v64 = _np(plan.rhs_vjp([t.double() for t in q32], [t.double() for t in y32]))
+    v32 = _np(plan.rhs_vjp(q32, y32))
+    torch.cuda.synchronize()
+    del plan
+    errs = field_errors(v32, v64)
+    print(f"  rel L2 per field: {['%.2e' % e for e in errs]}", flush=True)
+    assert max(errs) < 1e-6, errs
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16x3", 1e-4)])
