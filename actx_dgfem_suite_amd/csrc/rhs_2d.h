@@ -59,7 +59,7 @@ struct Euler2DFlux {
   __device__ static void normal_flux(const double* q, const double* n, double gamma,
                                      double* fn, double* lam) {
     const double rho = q[0], e = q[1];
-    const double inv_rho = 1.0 / rho;
+    const double inv_rho = rcp_nr(rho);
     const double ux = q[2] * inv_rho, uy = q[3] * inv_rho;
     const double p = (gamma - 1.0) * (e - 0.5 * (q[2] * ux + q[3] * uy));
     const double un = ux * n[0] + uy * n[1];
@@ -67,7 +67,7 @@ struct Euler2DFlux {
     fn[1] = (e + p) * un;
     fn[2] = q[2] * un + p * n[0];
     fn[3] = q[3] * un + p * n[1];
-    *lam = sqrt(ux * ux + uy * uy) + sqrt(gamma * p * inv_rho);
+    *lam = sqrt_nr(ux * ux + uy * uy) + sqrt_nr(gamma * p * inv_rho);
   }
   __device__ static void flux(const double* qm, const double* qp, const double* n,
                               double gamma, double* out) {
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void rhs_2d_kernel(Rhs2DArgs<T> a) {
       } else if constexpr (EQ == DG_EQ_EULER) {
         const double gamma = c0;
         const double rho = s_q[0][node], e = s_q[1][node];
-        const double inv_rho = 1.0 / rho;
+        const double inv_rho = rcp_nr(rho);
         const double m[2] = {s_q[2][node], s_q[3][node]};
         const double u[2] = {m[0] * inv_rho, m[1] * inv_rho};
         const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1]));

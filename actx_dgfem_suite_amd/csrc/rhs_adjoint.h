@@ -147,7 +147,7 @@ struct EulerAdj {
                                    const double* psi, double* g) {
     const double rho = q[0], e = q[1];
     const double m[3] = {q[2], q[3], q[4]};
-    const double inv = 1.0 / rho;
+    const double inv = rcp_nr(rho);
     const double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
     const double p = (gamma - 1.0) * (e - 0.5 * m2 * inv);
     const double un = (m[0] * n[0] + m[1] * n[1] + m[2] * n[2]) * inv;
@@ -164,16 +164,16 @@ struct EulerAdj {
   // wavespeed |u| + c and its gradient
   __device__ static double speed(const double* q, double gamma, double* ds) {
     const double rho = q[0], e = q[1];
-    const double inv = 1.0 / rho;
+    const double inv = rcp_nr(rho);
     const double u[3] = {q[2] * inv, q[3] * inv, q[4] * inv};
     const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const double p = (gamma - 1.0) * (e - 0.5 * rho * uu);
-    const double c = sqrt(gamma * p * inv);
-    const double au = sqrt(uu);
+    const double c = sqrt_nr(gamma * p * inv);
+    const double au = sqrt_nr(uu);
     const double dp_drho = (gamma - 1.0) * 0.5 * uu;
     // two reciprocals instead of a division per term
-    const double k = 0.5 * gamma * inv * (1.0 / c);
-    const double iau = au > 0.0 ? inv / au : 0.0;
+    const double k = 0.5 * gamma * inv * rcp_nr(c);
+    const double iau = au > 0.0 ? inv * rcp_nr(au) : 0.0;
     ds[0] = (au > 0.0 ? -au * inv : 0.0) + k * (dp_drho - p * inv);
     ds[1] = k * (gamma - 1.0);
 #pragma unroll
@@ -283,7 +283,7 @@ struct Euler2DAdj {
                                    const double* psi, double* g) {
     const double rho = q[0], e = q[1];
     const double m[2] = {q[2], q[3]};
-    const double inv = 1.0 / rho;
+    const double inv = rcp_nr(rho);
     const double m2 = m[0] * m[0] + m[1] * m[1];
     const double p = (gamma - 1.0) * (e - 0.5 * m2 * inv);
     const double un = (m[0] * n[0] + m[1] * n[1]) * inv;
@@ -300,15 +300,15 @@ struct Euler2DAdj {
   // wavespeed |u| + c and its gradient
   __device__ static double speed(const double* q, double gamma, double* ds) {
     const double rho = q[0], e = q[1];
-    const double inv = 1.0 / rho;
+    const double inv = rcp_nr(rho);
     const double u[2] = {q[2] * inv, q[3] * inv};
     const double uu = u[0] * u[0] + u[1] * u[1];
     const double p = (gamma - 1.0) * (e - 0.5 * rho * uu);
-    const double c = sqrt(gamma * p * inv);
-    const double au = sqrt(uu);
+    const double c = sqrt_nr(gamma * p * inv);
+    const double au = sqrt_nr(uu);
     const double dp_drho = (gamma - 1.0) * 0.5 * uu;
-    const double k = 0.5 * gamma * inv * (1.0 / c);
-    const double iau = au > 0.0 ? inv / au : 0.0;
+    const double k = 0.5 * gamma * inv * rcp_nr(c);
+    const double iau = au > 0.0 ? inv * rcp_nr(au) : 0.0;
     ds[0] = (au > 0.0 ? -au * inv : 0.0) + k * (dp_drho - p * inv);
     ds[1] = k * (gamma - 1.0);
 #pragma unroll

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void rhs_lds_kernel(RhsArgs a) {
       } else if constexpr (EQ == DG_EQ_EULER) {
         const double gamma = c0;
         const double rho = s_q[0][node], e = s_q[1][node];
-        const double inv_rho = 1.0 / rho;
+        const double inv_rho = rcp_nr(rho);
         const double m[3] = {s_q[2][node], s_q[3][node], s_q[4][node]};
         const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
         const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
               for (int x = 0; x < 3; ++x) stg[(2 + x) * 16 * NP + n * NP + j] = (float)m[x];
             }
-            const double inv_rho = 1.0 / rho;
+            const double inv_rho = rcp_nr(rho);
             const double u[3] = {m[0] * inv_rho, m[1] * inv_rho, m[2] * inv_rho};
             const double p = (gamma - 1.0) * (e - 0.5 * (m[0] * u[0] + m[1] * u[1] + m[2] * u[2]));
             const double h = (e + p) * inv_rho;
