@@ -210,9 +210,12 @@ __host__ __device__ constexpr int ref_face_coef(int f, int r) {
 // NF * 16 * Np doubles of LDS per wave.  Measured on MI355X, it pays where the
 // blocks per CU stay what the registers allow (wave P1 -16%, wave P2 -9%,
 // Euler P3 -7%, Maxwell P1 -18%, Maxwell P3 -11%, Maxwell P2 even) and not
-// where the LDS halves them (wave P3 +8%, Euler P2 +18%); at P4 it does not fit.
+// where the LDS halves them (wave P3 +8%, Euler P2 +18%).  At P4 it fits the
+// wave operator only (146 KB per block with the direct flux): FP64 wave P4
+// even, and with it the in-tile neighbours, which the FP32 mixed kernel
+// reads from the same map, FP32 wave P4 -9%.
 __host__ __device__ constexpr bool stage_own_traces_of(int eq, int order) {
-  return direct_flux_of(eq, order) ? order <= 3
+  return direct_flux_of(eq, order) ? (order <= 3 || eq == DG_EQ_WAVE)
                                    : (eq == DG_EQ_WAVE ? order <= 2
                                                        : (eq == DG_EQ_EULER ? order == 3 : order <= 3));
 }
