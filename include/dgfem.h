@@ -216,7 +216,8 @@ int dg_rhs_lserk_stage_mixed(const dg_plan* plan, int32_t precision, double a,
  * weak forms (see dg_adjoint_space): R M^{-1} rhs_bar into the workspace,
  * then the forward kernel with R M applied to its output (fused into the
  * MFMA kernel; a separate pass after the LDS-staged one).  Euler and
- * triangle plans: M_ref^{-1} rhs_bar, then the adjoint kernel.  q_bar may
+ * triangle plans: M_ref^{-1} rhs_bar, then an adjoint kernel (Euler on
+ * tetrahedra: the MFMA adjoint kernel on 16-element tiles).  q_bar may
  * be rhs_bar, but not the workspace or q: the second kernel gathers
  * neighbour values from those while it writes q_bar.  The reference has
  * no adjoint: its jax:jit array context (run/__main__.py:213) is where
