@@ -191,11 +191,11 @@ constexpr bool ref_normal_lift() { return ref_normal_lift_of(EQ, ORDER); }
 // instead of forming Flux::boundary and selecting per field, and (wave with
 // the reference-normal LIFT) evaluates the upwind flux from the two normal
 // velocities.  Fewer VALU instructions: measured on MI355X wave P2 -6%, P3
-// -3.6%, Maxwell P3 -2%; at wave P4 (one wave per SIMD) each measured +2-3%,
-// so it keeps the general forms there.
+// -3.6%, Maxwell P3 -2%, wave P4 -2.3% (with its staged state; before that
+// it measured +2-3% there).
 template <int EQ, int ORDER>
 constexpr bool lean_flux() {
-  return FluxOf<EQ>::type::mirror_sign(0) != 0 && !(EQ == DG_EQ_WAVE && ORDER == 4);
+  return FluxOf<EQ>::type::mirror_sign(0) != 0;
 }
 
 // c_{f,r}: reference face normal times area ratio (faces of the reference
