@@ -307,7 +307,10 @@ __device__ constexpr int levi(int o, int x, int a) {
 
 // Unroll depth of the volume k-step loops: enough independent MFMAs and loads
 // in flight without hoisting every load of the tile into registers.
-#define KUNROLL 3
+// Measured on MI355X against 3: Maxwell P4 -7.9%, wave P4 -7% to -10%, Euler
+// P4 -4% (nine k-steps; a full unroll measures the same), P1-P3 even; 1 is
+// 5-39% slower.
+#define KUNROLL 2
 // Unroll depth of the direct-flux k-step loop (measured: 3 against 2 and 4,
 // except Maxwell P4, 4).
 #define DUNROLL 3
