@@ -406,8 +406,11 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
       for (int b = 0; b < MB; ++b) acc[c][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // {{{ volume terms (B operands built in FP64, rounded once to FP32)
+    // (k-step loops unrolled by 2; measured on MI355X against no unrolling:
+    // FP32 wave P3 -9%, wave P4 -7.5%, Maxwell P3 -9%, Maxwell P4 -3.5%,
+    // Euler -3% to even; BF16x3 -3% to +1.6%)
 
-#pragma unroll 1
+#pragma unroll 2
     for (int kq = 0; kq < KS / KG; ++kq) {
       if constexpr (EQ == DG_EQ_WAVE) {
         float ub[KG], wb[3][KG];
@@ -573,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         for (int r = 0; r < 3; ++r)
 #pragma unroll
           for (int b = 0; b < MB; ++b) tr[r][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+#pragma unroll 2
         for (int kq = 0; kq < KS / KG; ++kq) {
           float ub[KG];
 #pragma unroll
