@@ -128,7 +128,9 @@ struct WaveFlux {
 // correctly rounded sequences with their range scaling.  Measured on gfx950
 // over 4.2M log-uniform samples in [6e-6, 1.6e5]: equal to the IEEE results
 // bit for bit, and the Euler RHS bit-identical; Euler P2 RHS -5%, its adjoint
-// -2% to -9% across P1-P4.  sqrt_nr(0) = 0.
+// -2% to -9% across P1-P4.  sqrt_nr(0) = 0, and a negative argument (a state
+// with negative pressure) gives NaN, as sqrt does (the LF flux's fmax then
+// takes the other side's wavespeed, as it does with sqrt).
 __device__ inline double rcp_nr(double x) {
   double r = __builtin_amdgcn_rcp(x);
   double e = __builtin_fma(-x, r, 1.0);
@@ -144,7 +146,7 @@ __device__ inline double sqrt_nr(double x) {
   h = __builtin_fma(h, r, h);
   const double d = __builtin_fma(-g, g, x);
   g = __builtin_fma(d, h, g);
-  return x > 0.0 ? g : 0.0;
+  return x == 0.0 ? 0.0 : g;
 }
 
 // Euler, local Lax-Friedrichs: F* = {F}.n - lam/2 (q+ - q-), lam = max(|u|+c)
