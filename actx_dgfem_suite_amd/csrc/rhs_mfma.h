@@ -790,8 +790,8 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
       if constexpr (ST) wave_lds_fence();
     } else {
     // flux chunks unrolled by 2 at Maxwell P3 (measured -6%; Euler P3 +12%,
-    // and +10% in the fused R M form of the adjoint)
-#pragma unroll((EQ == DG_EQ_MAXWELL && ORDER == 3 && !POST) ? 2 : 1)
+    // and +10% in the fused R M form of the adjoint, +8% with the fused stage)
+#pragma unroll((EQ == DG_EQ_MAXWELL && ORDER == 3 && !POST && !STAGE) ? 2 : 1)
     for (int ch = 0; ch < NCHUNK; ++ch) {
       // producer: items (face pair p, node kk) with kk fastest
 #pragma unroll

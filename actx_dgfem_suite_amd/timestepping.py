@@ -81,18 +81,21 @@ _FUSED_REVERSE = {("wave", 1), ("wave", 2), ("wave", 3), ("wave", 4), ("maxwell"
 def fused_stage_preferred(plan, precision: str = "fp64") -> bool:
     """Default of ``LSERK4(fused=None)``.  Mixed forms: fused stages on the
     tetrahedral cases in ``_FUSED_MIXED``.  FP64: fused stages where the plan
-    has them, except Maxwell P4.
+    has them, except Maxwell P3 and P4.
 
     FP64, measured on MI355X over 20 whole steps (``run``), fused against RHS +
-    stage kernels, on meshes of 94k-750k tets: 1.01-1.14x faster for wave and
-    Euler P1-P4 and Maxwell P1 and P3, 0.987x for Maxwell P4.  Unfused, the
-    stage kernel re-reads the state and RHS the RHS kernel has just touched
-    while they are still in the 256 MB Infinity Cache, which takes back most
-    of what fusing saves in HBM traffic for Maxwell P4's six fields."""
+    stage kernels, on meshes of 94k-750k tets: 1.01-1.23x faster for wave and
+    Euler P1-P4 and Maxwell P1; Maxwell P3 0.98x (its RHS kernel unrolls its
+    flux chunks, which the fused form does not gain from) and P4 0.99-1.01x.
+    Unfused, the stage kernel re-reads the state and RHS the RHS kernel has
+    just touched while they are still in the 256 MB Infinity Cache, which
+    takes back most of what fusing saves in HBM traffic for Maxwell P4's six
+    fields."""
     if precision != "fp64":
         return bool(plan.fused_stage_mixed) and \
             (plan.equation, plan.discr.order) in _FUSED_MIXED[precision]
-    return bool(plan.fused_stage) and not (plan.equation == "maxwell" and plan.discr.order == 4)
+    return bool(plan.fused_stage) and not (plan.equation == "maxwell" and
+                                           plan.discr.order in (3, 4))
 
 
 class LSERK4:
