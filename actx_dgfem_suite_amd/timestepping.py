@@ -56,26 +56,27 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
     return [t.clone() for t in fields]
 
 
-# Mixed forms whose fused step measured at least 4% faster on MI355X (10
-# whole steps through ``run`` on meshes of 94k-750k tets, fused against the
-# mixed RHS and stage kernels).  FP32: wave P1 1.10x, P3 1.08x, P4 1.14x,
-# Euler P1 1.06x.  BF16x3: wave P2 1.08x, P3 1.05x, P4 1.12x, Euler P3
-# 1.12x, Maxwell P3 1.07x, P4 1.06x.  Elsewhere it was slower (0.65x-0.99x;
-# Euler P2 BF16x3 1.02x), mostly where the stage epilogue's registers cost a
-# wave per SIMD.
+# Mixed forms whose fused step measured at least 4% faster on MI355X (20
+# whole steps through ``run`` on the 4M-DOF meshes, 94k-1.05M tets, fused
+# against the mixed RHS and stage kernels).  FP32: wave P1 1.14x, Euler P1
+# 1.09x, Maxwell P1 1.09x.  BF16x3: wave P1 1.06x, P2 1.08x, P4 1.06x on
+# 584k tets (1.01x on 118k).  Elsewhere 0.69x-1.03x: the unfused RHS kernels
+# unroll their volume loops by 2 and read in-tile neighbours from the FP64
+# kernel's map, and the stage epilogue's registers take that back in the
+# fused form (FP32 wave P4 0.82-0.85x, Maxwell P3 0.69x; BF16x3 wave P3
+# 0.83x).
 _FUSED_MIXED = {
-    "fp32": {("wave", 1), ("wave", 3), ("wave", 4), ("euler", 1)},
-    "bf16x3": {("wave", 2), ("wave", 3), ("wave", 4), ("euler", 3), ("maxwell", 3),
-               ("maxwell", 4)},
+    "fp32": {("wave", 1), ("euler", 1), ("maxwell", 1)},
+    "bf16x3": {("wave", 1), ("wave", 2), ("wave", 4)},
 }
 
 
 # Linear operators whose fused reverse step in the adjoint variables
 # (``dg_rhs_lserk_stage_adjoint``) measured at least 3% faster on MI355X
-# than ``dg_rhs`` + ``dg_lserk_stage_adjoint`` (reverse steps on meshes of
-# 94k-750k tets): wave P1 1.10x, P2 1.07x, P3 1.13x, P4 1.03x, Maxwell P2
-# 1.03x.  Maxwell P1 0.99x, P3 1.01x, P4 0.87x.
-_FUSED_REVERSE = {("wave", 1), ("wave", 2), ("wave", 3), ("wave", 4), ("maxwell", 2)}
+# than ``dg_rhs`` + ``dg_lserk_stage_adjoint`` (20 reverse steps on the
+# 4M-DOF meshes, 94k-1.05M tets): wave P1 1.24x, P3 1.16x, Maxwell P1
+# 1.23x, P2 1.05x.  Wave P2 1.01x, P4 0.99x, Maxwell P3 0.98x, P4 1.00x.
+_FUSED_REVERSE = {("wave", 1), ("wave", 3), ("maxwell", 1), ("maxwell", 2)}
 
 
 def fused_stage_preferred(plan, precision: str = "fp64") -> bool:
