@@ -931,6 +931,9 @@ struct dg_plan {
   void* scratch = nullptr;       // this device's `scratch` (dg_rhs_vjp_noalloc)
   double* ext_fscale = nullptr;  // (K, F)
   int32_t* node_faces = nullptr; // (Np, 3)
+  // whether the face geometry satisfies Fscale n = rx^T c_f (checked where a
+  // kernel of this plan relies on it)
+  bool ref_normals_ok = true;
   // FP64 Euler adjoint on tetrahedra: the MFMA kernel and its A tiles
   VjpMfmaKernel vjp_mfma_kernel = nullptr;
   int vjp_mfma_grid = 0;
@@ -1087,6 +1090,22 @@ extern "C" {
 
 const char* dg_last_error_string(void) { return g_last_error.c_str(); }
 
+namespace {
+
+// The mixed wave kernels on the direct flux LIFT the velocity flux in
+// reference normals: refused on a geometry that does not allow it.
+int check_mixed_geometry(const dg_plan* p, int32_t precision, const char* name) {
+  if (p->dim != 2 && mixed_direct_flux_of(p->equation, precision, p->order) && !p->ref_normals_ok)
+    return fail(DG_ERR_UNSUPPORTED,
+                std::string(name) +
+                    ": the mixed wave kernel LIFTs its velocity flux in reference normals, and "
+                    "this plan's face geometry is not the affine image of the reference "
+                    "element's (Fscale n != rx^T c_f); use the FP64 form");
+  return DG_OK;
+}
+
+}  // namespace
+
 int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, float* const* out,
                  void* stream) {
   if (!p) return fail(DG_ERR_INVALID_ARGUMENT, "NULL plan");
@@ -1094,6 +1113,7 @@ int dg_rhs_mixed(const dg_plan* p, int32_t precision, const float* const* in, fl
     return fail(DG_ERR_UNSUPPORTED, "mixed precision " + std::to_string(precision) +
                                         " not supported (FP32 or BF16X3)");
   int rc;
+  if ((rc = check_mixed_geometry(p, precision, "dg_rhs_mixed"))) return rc;
   if ((rc = check_ptrs((const void* const*)in, p->nfields, "fields_in")) ||
       (rc = check_ptrs((const void* const*)out, p->nfields, "fields_out")) ||
       (rc = check_disjoint((const void* const*)out, "fields_out",
@@ -1132,6 +1152,7 @@ int dg_rhs_lserk_stage_mixed(const dg_plan* p, int32_t precision, double a, doub
                 "dg_rhs_lserk_stage_mixed: tetrahedral plans only (use dg_rhs_mixed and "
                 "dg_lserk_stage_mixed)");
   int rc;
+  if ((rc = check_mixed_geometry(p, precision, "dg_rhs_lserk_stage_mixed"))) return rc;
   if ((rc = check_ptrs((const void* const*)q32_in, p->nfields, "q32_in")) ||
       (rc = check_ptrs((const void* const*)q32_out, p->nfields, "q32_out")) ||
       (rc = check_ptrs((const void* const*)q, p->nfields, "q")) ||
@@ -1303,8 +1324,15 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       }
       p->variant = DG_KERNEL_LDS;
     }
+    // the kernels that LIFT the wave operator's velocity flux in reference
+    // normals (the FP64 MFMA kernel; the mixed kernels where they run the
+    // direct flux) need Fscale n = rx^T c_f on every face
+    if (ref_normal_lift_of(d->equation, d->order) ||
+        mixed_direct_flux_of(d->equation, DG_PREC_FP32, d->order) ||
+        mixed_direct_flux_of(d->equation, DG_PREC_BF16X3, d->order))
+      p->ref_normals_ok = geometry_fits_ref_normals(d->nelements, d->geometry);
     if (p->variant == DG_KERNEL_MFMA && ref_normal_lift_of(d->equation, d->order) &&
-        !geometry_fits_ref_normals(d->nelements, d->geometry)) {
+        !p->ref_normals_ok) {
       if (d->variant == DG_KERNEL_MFMA) {
         free_plan(p);
         return fail(DG_ERR_INVALID_ARGUMENT,

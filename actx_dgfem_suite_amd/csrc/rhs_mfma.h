@@ -447,7 +447,7 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
       for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
     }
-    if constexpr (FG) {
+    if constexpr (FG && !RN) {
       const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
       const double2 f0 = src[lane], f1 = src[lane + 64];
       fgl[lane] = f0;
@@ -460,6 +460,25 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
     double rx[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) rx[k] = PV ? rxv[k] : a.vgeo[(tile * 9 + k) * 16 + n];
+    if constexpr (FG && RN) {
+      // The tile's face geometry from rx instead of the plan's table: lane
+      // (g, n) forms face g of element n as Fscale n = rx^T c_g, which the
+      // plan checks the geometry satisfies before it runs the reference-normal
+      // LIFT (within rounding of the table's values).  Measured on MI355X:
+      // wave P3 -3% (162k tets), -3.9% (216k tets), wave P4 -0.8%.
+      double m[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const double c0 = g == 3 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        const double c1 = g == 1 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        const double c2 = g == 0 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        m[x] = c0 * rx[3 * x] + c1 * rx[3 * x + 1] + c2 * rx[3 * x + 2];
+      }
+      const double fs = sqrt_nr(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+      const double inv = rcp_nr(fs);
+      fgl[2 * (g * 16 + n)] = double2{m[0] * inv, m[1] * inv};
+      fgl[2 * (g * 16 + n) + 1] = double2{m[2] * inv, fs};
+    }
     // state at volume B-operand node j = g * KS + ks (callers check j < NP)
     auto qload = [&](int c, int ks) -> double {
       if constexpr (PV) return qv[c][ks];

@@ -425,6 +425,13 @@ def test_mfma_wave_plan_checks_face_geometry(device):
         DGPlan(bent, "wave", 1.0, variant="mfma")
     plan = DGPlan(bent, "wave", 1.0, variant="auto")
     assert plan.kernel_variant == "lds"
+    # the mixed wave kernels on the direct flux rely on the same identity:
+    # refused on this plan, run on the unaltered one
+    q32 = [torch.zeros(d.nelements, d.np, dtype=torch.float32, device=device)
+           for _ in range(4)]
+    for precision in ("fp32", "bf16x3"):
+        with pytest.raises(NotImplementedError, match="reference normals"):
+            plan.rhs(q32, precision=precision)
     # the unaltered discretization keeps the MFMA kernel
     assert DGPlan(d, "wave", 1.0, variant="auto").kernel_variant == "mfma"
     # the LDS-staged kernel evaluates the altered operator: compare it with a
