@@ -382,11 +382,16 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     double2* const fgl = s_fgeo[tid >> 6];
     constexpr bool MAPA = DFG && PREC == DG_PREC_FP32;   // map entries preloaded
     int32_t ext_all[MAPA ? KSD : 1];
+    // face geometry: BF16x3 copies the tile's table into LDS (forming it from
+    // rx measured +1.7% there), FP32 forms it from rx below
+    constexpr bool DFG_RX = DFG && PREC == DG_PREC_FP32;
     if constexpr (DFG) {
-      const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
-      const double2 f0 = src[lane], f1 = src[lane + 64];
-      fgl[lane] = f0;
-      fgl[lane + 64] = f1;
+      if constexpr (!DFG_RX) {
+        const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
+        const double2 f0 = src[lane], f1 = src[lane + 64];
+        fgl[lane] = f0;
+        fgl[lane + 64] = f1;
+      }
       if constexpr (MAPA)
 #pragma unroll
         for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
@@ -398,6 +403,24 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     double rx[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) rx[k] = TT ? a.vgeo[(tile * 9 + k) * 16 + n] : a.geo[el * kGeo + k];
+    if constexpr (DFG_RX) {
+      // the tile's face geometry from rx, Fscale n = rx^T c_g for face g of
+      // element n (lane (g, n)), as the FP64 kernel forms it; the plan
+      // refuses this kernel on geometry that does not satisfy it.  Measured
+      // on MI355X: FP32 wave P3 -8% (162k tets), -13% (216k tets)
+      double m[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const double c0 = g == 3 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        const double c1 = g == 1 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        const double c2 = g == 0 ? -1.0 : (g == 2 ? 1.0 : 0.0);
+        m[x] = c0 * rx[3 * x] + c1 * rx[3 * x + 1] + c2 * rx[3 * x + 2];
+      }
+      const double fs = sqrt_nr(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+      const double inv = rcp_nr(fs);
+      fgl[2 * (g * 16 + n)] = double2{m[0] * inv, m[1] * inv};
+      fgl[2 * (g * 16 + n) + 1] = double2{m[2] * inv, fs};
+    }
 
     f32x4 acc[NF][MB];
 #pragma unroll
