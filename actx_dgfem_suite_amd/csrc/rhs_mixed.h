@@ -335,7 +335,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // direct flux at P3: the tile's face geometry [face * 16 + n][4] in LDS and
   // every k-step's map entry loaded at the tile's start (measured: FP32 wave
   // P3 -10%; at P2 +5% and at P4 even, where its LDS costs a block per CU)
-  constexpr bool DFG = DFM && ORDER == 3;
+  // the tile's face geometry and exterior map held for the whole tile (DFG):
+  // P3, and FP32 P2 with the geometry formed from rx (FP32 P2 -11.5%; FP32
+  // P4 +16%)
+  constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32));
   __shared__ double2 s_fgeo[kBlock / 64][DFG ? 128 : 1];
   __shared__ int s_fmask[NFN];
   __shared__ float s_buf[WAVES][WB];
