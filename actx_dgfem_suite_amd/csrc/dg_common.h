@@ -51,6 +51,9 @@ struct RhsArgs {
   // tiles, the last one padded with its final element):
   const double* vgeo;        // [tile][9][16]: rx of element n of the tile
   const double* fgeo;        // [tile][face*16 + n][4]: n_x, n_y, n_z, Fscale
+  // 1 when every face satisfies Fscale n = rx^T c_f (affine images of the
+  // reference element), so that the kernel may form its face geometry from rx
+  int32_t face_geo_rx;
   const int32_t* ext_t;      // [tile][chunk][item][64]: emap in flux-item order
   const int32_t* ext_d;      // [tile][kstep][64]: emap in the direct flux order
   // MFMA kernel with the mass post-multiplication (the linear operators'

@@ -1055,6 +1055,7 @@ RhsArgs rhs_args(const dg_plan* p, const double* const* in, double* const* out, 
   a.lift_tiles = p->lift_tiles;
   a.vgeo = p->vgeo;
   a.fgeo = p->fgeo;
+  a.face_geo_rx = p->ref_normals_ok ? 1 : 0;
   a.ext_t = p->ext_t;
   a.ext_d = p->ext_d;
   a.nelem = p->nelem;
@@ -1326,11 +1327,9 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     }
     // the kernels that LIFT the wave operator's velocity flux in reference
     // normals (the FP64 MFMA kernel; the mixed kernels where they run the
-    // direct flux) need Fscale n = rx^T c_f on every face
-    if (ref_normal_lift_of(d->equation, d->order) ||
-        mixed_direct_flux_of(d->equation, DG_PREC_FP32, d->order) ||
-        mixed_direct_flux_of(d->equation, DG_PREC_BF16X3, d->order))
-      p->ref_normals_ok = geometry_fits_ref_normals(d->nelements, d->geometry);
+    // direct flux) need Fscale n = rx^T c_f on every face; the MFMA kernel
+    // of the other operators forms its face geometry from rx where it holds
+    p->ref_normals_ok = geometry_fits_ref_normals(d->nelements, d->geometry);
     if (p->variant == DG_KERNEL_MFMA && ref_normal_lift_of(d->equation, d->order) &&
         !p->ref_normals_ok) {
       if (d->variant == DG_KERNEL_MFMA) {

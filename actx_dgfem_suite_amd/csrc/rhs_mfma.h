@@ -447,11 +447,18 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
 #pragma unroll
       for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
     }
-    if constexpr (FG && !RN) {
-      const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
-      const double2 f0 = src[lane], f1 = src[lane + 64];
-      fgl[lane] = f0;
-      fgl[lane + 64] = f1;
+    // the tile's face geometry into LDS: formed from rx where the plan found
+    // Fscale n = rx^T c_f on every face (always with the reference-normal
+    // LIFT, which needs it), else copied from the plan's table
+    constexpr bool FGRX = FG && !(EQ == DG_EQ_MAXWELL && ORDER == 4);
+    const bool geo_rx = FGRX && (RN || a.face_geo_rx != 0);
+    if constexpr (FG) {
+      if (!geo_rx) {
+        const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
+        const double2 f0 = src[lane], f1 = src[lane + 64];
+        fgl[lane] = f0;
+        fgl[lane + 64] = f1;
+      }
     }
     if constexpr (PF) load_ext(tile, 0);
     const int64_t gel = tile * 16 + n;
@@ -460,12 +467,13 @@ __global__ __launch_bounds__(64 * mfma_waves_of(EQ, ORDER)) void rhs_mfma_kernel
     double rx[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) rx[k] = PV ? rxv[k] : a.vgeo[(tile * 9 + k) * 16 + n];
-    if constexpr (FG && RN) {
+    if constexpr (FGRX) if (geo_rx) {
       // The tile's face geometry from rx instead of the plan's table: lane
-      // (g, n) forms face g of element n as Fscale n = rx^T c_g, which the
-      // plan checks the geometry satisfies before it runs the reference-normal
-      // LIFT (within rounding of the table's values).  Measured on MI355X:
-      // wave P3 -3% (162k tets), -3.9% (216k tets), wave P4 -0.8%.
+      // (g, n) forms face g of element n as Fscale n = rx^T c_g (within
+      // rounding of the table's values).  Measured on MI355X: wave P3 -3%
+      // (162k tets), -3.9% (216k tets), wave P4 -0.8%, Euler P1 -10%, P2
+      // -5.5%, P3-P4 -1%, Maxwell P2 -12.5%, P3 -10%; Maxwell P4 +1.2%, so
+      // it copies the table there.
       double m[3];
 #pragma unroll
       for (int x = 0; x < 3; ++x) {

@@ -401,6 +401,39 @@ def test_rhs_single_partial_tile(equation, order, variant, device):
             assert max(field_errors(m, ref)) < 1e-3, prec
 
 
+@pytest.mark.parametrize("equation,order", [("euler", 1), ("euler", 2), ("maxwell", 2),
+                                            ("maxwell", 3), ("wave", 3)])
+def test_mfma_face_geometry_from_rx_or_table(equation, order, device):
+    """The MFMA kernel forms its tiles' face geometry from rx where every
+    face satisfies Fscale n_f = rx^T c_f, and copies the plan's table where
+    one does not (Euler and Maxwell, which take any geometry on this kernel).
+    Both match a host evaluation of the same fused forms on their geometry;
+    the wave plan on the altered geometry goes to the LDS-staged kernel."""
+    print(f"[case] face geometry from rx or table {equation} P{order}", flush=True)
+    from oracle import dg_cpu
+    d = make_box_discretization(order, 3)
+    geo = d.packed_geometry().copy()
+    geo[7, 9 + 4 * 2 + 3] *= 1.5          # one face's Fscale off the identity
+
+    class Bent:
+        def __getattr__(self, name):
+            return getattr(d, name)
+
+        def packed_geometry(self):
+            return geo
+
+    q = random_state(equation, d.nodes, seed=order)
+    for discr in (d, Bent()):
+        plan = DGPlan(discr, equation, PARAM[equation], variant="auto")
+        expect = "lds" if (equation == "wave" and discr is not d) else "mfma"
+        assert plan.kernel_variant == expect
+        got = _run(plan, q, device)
+        ref = dg_cpu.CpuOperator(discr, equation, PARAM[equation])(q)
+        err = max(field_errors(got, ref))
+        print(f"  {'bent' if discr is not d else 'affine'}: {plan.kernel_variant} {err:.2e}", flush=True)
+        assert err < TOL_FP64
+
+
 def test_mfma_wave_plan_checks_face_geometry(device):
     """The MFMA wave kernel LIFTs the velocity flux in reference normals,
     which holds when every element's face geometry is the affine image of
