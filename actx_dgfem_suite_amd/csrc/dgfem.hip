@@ -1076,6 +1076,7 @@ MixedArgs mixed_args(const dg_plan* p, int k, const float* const* in, float* con
   a.fmask = p->fmask;
   a.vgeo = p->vgeo;
   a.fgeo = p->fgeo;
+  a.face_geo_rx = p->ref_normals_ok ? 1 : 0;
   a.ext_t = p->ext_t;
   a.ext_d = p->ext_d;
   a.op_tiles = p->mixed_op[k];

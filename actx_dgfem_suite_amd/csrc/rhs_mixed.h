@@ -50,6 +50,7 @@ struct MixedArgs {
   // geometry, and the exterior map in that kernel's flux-item order
   const double* vgeo;       // [tile][9][16]
   const double* fgeo;       // [tile][face*16 + n][4]
+  int32_t face_geo_rx;      // 1: every face satisfies Fscale n = rx^T c_f
   const int32_t* ext_t;     // [tile][chunk][item][64]
   const int32_t* ext_d;     // [tile][kstep][64]: the direct flux order (in-tile entries marked)
   const void* op_tiles;     // A tiles of D (float, or bf16 hi/lo pairs)
@@ -306,6 +307,25 @@ __host__ __device__ constexpr int mixed_pair_of(int eq, int prec, int order, int
                               : (16 * kq + 4 * g + i < 4 * nfp_of(order) ? 16 * kq + 4 * g + i : -1);
 }
 
+// Whether the flux producer off the direct flux reads its face geometry
+// from a per-wave LDS copy of the tile's, formed from rx (Fscale n = rx^T
+// c_f, where the plan finds it on every face; else copied from the table)
+// instead of two global loads per flux item.  Measured on MI355X, the cases
+// that gained: FP32 Maxwell P1 -24%, wave P1 -16%, Euler P3 -7%, Maxwell P2
+// -4.6%, P4 -4.2%, Euler P1 -2%; BF16x3 Maxwell P1 -17%, wave P2 -7.5%,
+// Maxwell P4 -6.6%, Euler P4 -3.7%, Maxwell P2 -2%.  Elsewhere up to +64%
+// (FP32 Maxwell P3; BF16x3 wave P1 +14%), where its LDS or registers cost a
+// wave.
+template <int EQ, int PREC, int ORDER>
+constexpr bool mixed_face_geo_lds() {
+  if constexpr (PREC == DG_PREC_FP32)
+    return (EQ == DG_EQ_WAVE && ORDER == 1) || (EQ == DG_EQ_EULER && (ORDER == 1 || ORDER == 3)) ||
+           (EQ == DG_EQ_MAXWELL && ORDER != 3);
+  else
+    return (EQ == DG_EQ_WAVE && ORDER == 2) || (EQ == DG_EQ_EULER && ORDER == 4) ||
+           (EQ == DG_EQ_MAXWELL && ORDER != 3);
+}
+
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
 // res = a res + dt rhs, q = q + b res, q32 = (float)q to it in its output
 // phase (the FP32 RHS tile is the one the unfused path would store).
@@ -339,7 +359,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // P3, and FP32 P2 with the geometry formed from rx (FP32 P2 -11.5%; FP32
   // P4 +16%)
   constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32));
-  __shared__ double2 s_fgeo[kBlock / 64][DFG ? 128 : 1];
+  constexpr bool FGM = !DFM && mixed_face_geo_lds<EQ, PREC, ORDER>();
+  constexpr bool FGL = DFG || FGM;   // the tile's face geometry in LDS
+  __shared__ double2 s_fgeo[kBlock / 64][FGL ? 128 : 1];
   __shared__ int s_fmask[NFN];
   __shared__ float s_buf[WAVES][WB];
   // (the direct flux has no flux buffer and always stages the tile)
@@ -385,16 +407,22 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     double2* const fgl = s_fgeo[tid >> 6];
     constexpr bool MAPA = DFG && PREC == DG_PREC_FP32;   // map entries preloaded
     int32_t ext_all[MAPA ? KSD : 1];
-    // face geometry: BF16x3 copies the tile's table into LDS (forming it from
-    // rx measured +1.7% there), FP32 forms it from rx below
+    // face geometry in LDS: on the direct flux, BF16x3 copies the tile's
+    // table (forming it from rx measured +1.7% there) and FP32 forms it from
+    // rx below (the plan refuses that kernel on other geometry); off it
+    // (FGM), formed from rx where the plan found Fscale n = rx^T c_f on
+    // every face, else copied
     constexpr bool DFG_RX = DFG && PREC == DG_PREC_FP32;
-    if constexpr (DFG) {
-      if constexpr (!DFG_RX) {
+    const bool geo_rx = DFG_RX || (FGM && a.face_geo_rx != 0);
+    if constexpr (FGL) {
+      if (!geo_rx) {
         const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
         const double2 f0 = src[lane], f1 = src[lane + 64];
         fgl[lane] = f0;
         fgl[lane + 64] = f1;
       }
+    }
+    if constexpr (DFG) {
       if constexpr (MAPA)
 #pragma unroll
         for (int ks = 0; ks < KSD; ++ks) ext_all[ks] = a.ext_d[(tile * KSD + ks) * 64 + lane];
@@ -406,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     double rx[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) rx[k] = TT ? a.vgeo[(tile * 9 + k) * 16 + n] : a.geo[el * kGeo + k];
-    if constexpr (DFG_RX) {
+    if constexpr (FGL) if (geo_rx) {
       // the tile's face geometry from rx, Fscale n = rx^T c_g for face g of
       // element n (lane (g, n)), as the FP64 kernel forms it; the plan
       // refuses this kernel on geometry that does not satisfy it.  Measured
@@ -650,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 
     // {{{ surface terms: flux in FP64, staged as FP32 B operands
 
-    if constexpr (ST) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // staged state complete
+    if constexpr (ST || FGL) asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // staged state / face geometry complete
 
     if constexpr (DFM) {
       // FP32 wave, the FP64 kernel's direct flux: at k-step ks lane (g, n)
@@ -854,7 +882,13 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
         int64_t pe = tile * 16 + fe;
         pe = pe < a.nelem ? pe : a.nelem - 1;
         double nrm[3], fscale;
-        if constexpr (TT) {
+        if constexpr (FGM) {
+          const double2 g01 = fgl[2 * p], g23 = fgl[2 * p + 1];
+          nrm[0] = g01.x;
+          nrm[1] = g01.y;
+          nrm[2] = g23.x;
+          fscale = g23.y;
+        } else if constexpr (TT) {
           const double2* pgeo = reinterpret_cast<const double2*>(a.fgeo + (tile * 64 + p) * 4);
           const double2 g01 = pgeo[0], g23 = pgeo[1];
           nrm[0] = g01.x;

@@ -407,7 +407,8 @@ def test_mfma_face_geometry_from_rx_or_table(equation, order, device):
     """The MFMA kernel forms its tiles' face geometry from rx where every
     face satisfies Fscale n_f = rx^T c_f, and copies the plan's table where
     one does not (Euler and Maxwell, which take any geometry on this kernel).
-    Both match a host evaluation of the same fused forms on their geometry;
+    Both match a host evaluation of the same fused forms on their geometry,
+    and so do the mixed forms (their kernels form or copy it the same way);
     the wave plan on the altered geometry goes to the LDS-staged kernel."""
     print(f"[case] face geometry from rx or table {equation} P{order}", flush=True)
     from oracle import dg_cpu
@@ -432,6 +433,17 @@ def test_mfma_face_geometry_from_rx_or_table(equation, order, device):
         err = max(field_errors(got, ref))
         print(f"  {'bent' if discr is not d else 'affine'}: {plan.kernel_variant} {err:.2e}", flush=True)
         assert err < TOL_FP64
+        if equation == "wave":
+            continue
+        # the mixed kernels form or copy their face geometry the same way
+        q32 = [torch.from_numpy(np.ascontiguousarray(f)).to(device).float() for f in q]
+        for precision, tol in (("fp32", 1e-5), ("bf16x3", 1e-4)):
+            out = plan.rhs(q32, precision=precision)
+            torch.cuda.synchronize()
+            gm = np.stack([t.double().cpu().numpy() for t in out])
+            em = max(field_errors(gm, ref))
+            print(f"    {precision}: {em:.2e}", flush=True)
+            assert em < tol
 
 
 def test_mfma_wave_plan_checks_face_geometry(device):
