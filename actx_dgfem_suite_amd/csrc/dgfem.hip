@@ -463,9 +463,12 @@ VjpMfmaKernel pick_vjp_mfma(int order) {
 //           Mf_f[b][a] where fmask_f[b] = j: the same contraction on volume nodes.
 // The exterior map per tile, lane and face node: ext_v[t][l][f * KSF + v] =
 // ext_map[e][f * Nfp + a] for a = l / 16 + 4 v < Nfp (element e = 16 t + l % 16,
-// the last one repeated past the end), else -1.
+// the last one repeated past the end), else -1; with `intile`
+// (vjp_mfma_intile), a neighbour inside the tile as kInTile | local element
+// << 8 | node.
 void build_vjp_mfma_tables(int64_t k, int np, int nfp, const double* amat_t, const double* fm,
-                           const int32_t* fmask, const int32_t* emap, std::vector<double>& vol,
+                           const int32_t* fmask, const int32_t* emap, bool intile,
+                           std::vector<double>& vol,
                            std::vector<double>& face, std::vector<double>& own,
                            std::vector<int32_t>& ext_v) {
   const int mb = (np + 15) / 16, ks_n = (np + 3) / 4, ksf = (nfp + 3) / 4;
@@ -509,8 +512,12 @@ void build_vjp_mfma_tables(int64_t k, int np, int nfp, const double* amat_t, con
       for (int f = 0; f < kFaces; ++f)
         for (int v = 0; v < ksf; ++v) {
           const int aa = (l >> 4) + 4 * v;
-          if (aa < nfp)
-            ext_v[((size_t)t * 64 + l) * next + f * ksf + v] = emap[e * nfn + f * nfp + aa];
+          if (aa >= nfp) continue;
+          int32_t x = emap[e * nfn + f * nfp + aa];
+          // neighbours inside the tile: kInTile | local element << 8 | node
+          if (intile && x >= 0 && x / np / 16 == t)
+            x = kInTile | (int32_t)(((x / np) % 16) << 8) | (x % np);
+          ext_v[((size_t)t * 64 + l) * next + f * ksf + v] = x;
         }
     }
 }
@@ -1535,8 +1542,8 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
     if (!tri && d->equation == DG_EQ_EULER) {
       std::vector<double> vt, ft, ot;
       std::vector<int32_t> xv;
-      build_vjp_mfma_tables(k, np, nfp, amat_t.data(), fm.data(), d->face_nodes, d->ext_map, vt,
-                            ft, ot, xv);
+      build_vjp_mfma_tables(k, np, nfp, amat_t.data(), fm.data(), d->face_nodes, d->ext_map,
+                            vjp_mfma_intile(d->order), vt, ft, ot, xv);
       if ((rc = upload(&p->vjp_vol_tiles, vt.data(), vt.size())) ||
           (rc = upload(&p->vjp_face_tiles, ft.data(), ft.size())) ||
           (rc = upload(&p->vjp_own_tiles, ot.data(), ot.size())) ||
@@ -1861,6 +1868,7 @@ int rhs_vjp_impl(const dg_plan* p, const T* const* q, const T* const* rhs_bar, T
       v.ext_v = p->vjp_ext;
       v.fmask = p->fmask;
       v.ext_fscale = p->ext_fscale;
+      v.face_geo_rx = p->ref_normals_ok ? 1 : 0;
       v.vol_tiles = p->vjp_vol_tiles;
       v.face_tiles = p->vjp_face_tiles;
       v.own_tiles = p->vjp_own_tiles;

@@ -49,6 +49,7 @@ struct VjpMfmaArgs {
   const int32_t* ext_v;          // tile-ordered exterior map: [t][lane][f * KSF + v] (-1: none)
   const int32_t* fmask;          // (4 Nfp)
   const double* ext_fscale;      // (K, 4) Fscale of the neighbour across each face
+  int32_t face_geo_rx;           // 1: every face satisfies Fscale n = rx^T c_f
   const double* vol_tiles;       // A tiles of A_r: [(r * MB + b) * KS + ks][lane]
   const double* face_tiles;      // A tiles of Mf_f: [f * KSF + ks][lane]
   const double* own_tiles;       // A tiles of P_f: [f * KS + ks][lane]
@@ -75,6 +76,10 @@ struct VjpMfmaShape {
   static constexpr bool STQ = ORDER <= 3;
   static constexpr int EXT = kFaces * KSF;     // exterior-map entries per lane
 };
+
+// Whether the exterior map marks neighbours inside the tile, whose state
+// the kernel then reads from its staged copy (P2; needs the staged state).
+__host__ __device__ constexpr bool vjp_mfma_intile(int order) { return order == 2; }
 
 // Blocks per CU the registers must allow: two (two waves per SIMD) where the
 // kernel fits them without spilling (P1, P2), else one.
@@ -234,10 +239,30 @@ __global__ __launch_bounds__(256, vjp_mfma_min_blocks(ORDER)) void rhs_vjp_mfma_
 
 #pragma unroll 1
     for (int f = 0; f < kFaces; ++f) {
-      const double2* fg = reinterpret_cast<const double2*>(a.fgeo) + (tile * 64 + f * 16 + n) * 2;
-      const double2 g01 = fg[0], g23 = fg[1];
-      const double nrm[3] = {g01.x, g01.y, g23.x};
-      const double fs = g23.y;
+      // face f of element n: n and Fscale formed from rx (Fscale n = rx^T
+      // c_f) where the plan found that on every face, else from the table
+      double nrm[3], fs;
+      if (a.face_geo_rx) {
+        double m[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const double c0 = f == 3 ? -1.0 : (f == 2 ? 1.0 : 0.0);
+          const double c1 = f == 1 ? -1.0 : (f == 2 ? 1.0 : 0.0);
+          const double c2 = f == 0 ? -1.0 : (f == 2 ? 1.0 : 0.0);
+          m[x] = c0 * rx[3 * x] + c1 * rx[3 * x + 1] + c2 * rx[3 * x + 2];
+        }
+        fs = sqrt_nr(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        const double inv = rcp_nr(fs);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) nrm[x] = m[x] * inv;
+      } else {
+        const double2* fg = reinterpret_cast<const double2*>(a.fgeo) + (tile * 64 + f * 16 + n) * 2;
+        const double2 g01 = fg[0], g23 = fg[1];
+        nrm[0] = g01.x;
+        nrm[1] = g01.y;
+        nrm[2] = g23.x;
+        fs = g23.y;
+      }
       const double xfs = a.ext_fscale[el * kFaces + f];
       // the face's exterior entries (a runtime face index into ext[] would
       // put it in scratch memory)
@@ -249,16 +274,34 @@ __global__ __launch_bounds__(256, vjp_mfma_min_blocks(ORDER)) void rhs_vjp_mfma_
         for (int f2 = 1; f2 < kFaces; ++f2)
           if (f == f2) fe[v] = ext[f2 * KSF + v];
       }
+      // Neighbours inside the tile (IT: kInTile | local element << 8 | node)
+      // take q from the staged tile, the others by buffer loads whose
+      // out-of-range lanes (those, and boundary nodes) touch no memory; mu
+      // of an in-tile neighbour is loaded at its global index.  Measured on
+      // MI355X: P2 -1.2%, P1 +1.4%, P3 +28%, so at P2 only.
+      constexpr bool IT = vjp_mfma_intile(ORDER);
       double mx[NF][KSF], qx[NF][KSF], mo[STQ ? 1 : NF][KSF];
 #pragma unroll
       for (int v = 0; v < KSF; ++v) {
-        const uint32_t off = fe[v] >= 0 ? (uint32_t)fe[v] * 8u : field_bytes;
+        const int32_t e = fe[v];
+        const bool loc = IT && e >= kInTile;
+        const bool glob = e >= 0 && !loc;
+        const int lel = (e >> 8) & 15, lnd = e & 255;
+        const uint32_t goff = glob ? (uint32_t)e * 8u : field_bytes;
+        const uint32_t moff =
+            loc ? (uint32_t)(((int64_t)tile * 16 + lel) * NP + lnd) * 8u : goff;
         const bool ok = g + 4 * v < NFP;
         const int nd = ok ? s_fmask[f * NFP + g + 4 * v] : 0;
 #pragma unroll
         for (int c = 0; c < NF; ++c) {
-          mx[c][v] = gather(rs_mu[c], off);
-          qx[c][v] = gather(rs_q[c], off);
+          mx[c][v] = gather(rs_mu[c], moff);
+          const double gv = gather(rs_q[c], goff);
+          if constexpr (IT) {
+            const double lv = sq[c * 16 * SP + (loc ? lel * SP + lnd : 0)];
+            qx[c][v] = loc ? lv : gv;
+          } else {
+            qx[c][v] = gv;
+          }
           if constexpr (!STQ) mo[c][v] = ok ? a.mu[c][el * NP + nd] : 0.0;
         }
       }
