@@ -56,17 +56,19 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
     return [t.clone() for t in fields]
 
 
-# Mixed forms whose fused step measured at least 4% faster on MI355X (20
-# whole steps through ``run`` on the 4M-DOF meshes, 94k-1.05M tets, fused
-# against the mixed RHS and stage kernels).  FP32: wave P1 1.14x, Euler P1
-# 1.09x, Maxwell P1 1.09x.  BF16x3: wave P1 1.06x, P2 1.08x, P4 1.06x on
-# 584k tets (1.01x on 118k).  Elsewhere 0.69x-1.03x: the unfused RHS kernels
-# unroll their volume loops by 2 and read in-tile neighbours from the FP64
-# kernel's map, and the stage epilogue's registers take that back in the
-# fused form (FP32 wave P4 0.82-0.85x, Maxwell P3 0.69x; BF16x3 wave P3
-# 0.83x).
+# Forms whose fused step measured faster on MI355X (20 whole steps through
+# ``run`` on the 4M-DOF meshes, 94k-1.05M tets, fused against the RHS and
+# stage kernels).  FP64, at least 1%: wave P1 1.25x, P3 1.19x, Maxwell P1
+# 1.24x, P2 1.01x, Euler P2 1.02x; wave P2 1.00x, P4 0.98x, Euler P1 0.93x,
+# P3 0.95x, P4 1.01x, Maxwell P3 1.00x, P4 1.00x.  Mixed, at least 4%: FP32
+# wave P1 1.10x; BF16x3 wave P1 1.06x, P2 1.04x, P4 1.06x on 584k tets (1.00x
+# on 118k); elsewhere 0.63x-1.02x.  Unfused, the stage kernel re-reads the
+# state and RHS that the RHS kernel has just touched while they are still in
+# the 256 MB Infinity Cache, and the fused forms' stage epilogue costs
+# registers their RHS kernels are short of.
+_FUSED_FP64 = {("wave", 1), ("wave", 3), ("euler", 2), ("maxwell", 1), ("maxwell", 2)}
 _FUSED_MIXED = {
-    "fp32": {("wave", 1), ("euler", 1), ("maxwell", 1)},
+    "fp32": {("wave", 1)},
     "bf16x3": {("wave", 1), ("wave", 2), ("wave", 4)},
 }
 
@@ -80,23 +82,13 @@ _FUSED_REVERSE = {("wave", 1), ("wave", 3), ("maxwell", 1), ("maxwell", 2)}
 
 
 def fused_stage_preferred(plan, precision: str = "fp64") -> bool:
-    """Default of ``LSERK4(fused=None)``.  Mixed forms: fused stages on the
-    tetrahedral cases in ``_FUSED_MIXED``.  FP64: fused stages where the plan
-    has them, except Maxwell P3 and P4.
-
-    FP64, measured on MI355X over 20 whole steps (``run``), fused against RHS +
-    stage kernels, on meshes of 94k-750k tets: 1.01-1.23x faster for wave and
-    Euler P1-P4 and Maxwell P1; Maxwell P3 0.98x (its RHS kernel unrolls its
-    flux chunks, which the fused form does not gain from) and P4 0.99-1.01x.
-    Unfused, the stage kernel re-reads the state and RHS the RHS kernel has
-    just touched while they are still in the 256 MB Infinity Cache, which
-    takes back most of what fusing saves in HBM traffic for Maxwell P4's six
-    fields."""
+    """Default of ``LSERK4(fused=None)``: fused stages on the tetrahedral
+    cases of ``_FUSED_FP64`` (FP64) and ``_FUSED_MIXED`` (mixed forms), where
+    the plan has them."""
     if precision != "fp64":
         return bool(plan.fused_stage_mixed) and \
             (plan.equation, plan.discr.order) in _FUSED_MIXED[precision]
-    return bool(plan.fused_stage) and not (plan.equation == "maxwell" and
-                                           plan.discr.order in (3, 4))
+    return bool(plan.fused_stage) and (plan.equation, plan.discr.order) in _FUSED_FP64
 
 
 class LSERK4:
