@@ -277,8 +277,12 @@ def test_fused_lserk_stage_rejects_misuse(device):
     tri = DGPlan(d2, "wave", 1.0)
     with pytest.raises(ValueError, match="fused"):
         LSERK4(tri, 1e-3, precision="fp32", fused=True)
-    assert LSERK4(plan, 1e-3).fused and not LSERK4(lds, 1e-3).fused
-    assert not LSERK4(plan, 1e-3, fused=False).fused
+    # the default follows the measured table: wave P1 fused, wave P2 not
+    d1, _ = discr_pair(1, 2)
+    p1 = DGPlan(d1, "wave", 1.0, variant="mfma")
+    assert LSERK4(p1, 1e-3).fused and not LSERK4(lds, 1e-3).fused
+    assert plan.fused_stage and not LSERK4(plan, 1e-3).fused
+    assert not LSERK4(p1, 1e-3, fused=False).fused
     # Maxwell P4 defaults to unfused stages (measured faster there)
     d4, _ = discr_pair(4, 2)
     mx4 = DGPlan(d4, "maxwell", 0.5, variant="mfma")
