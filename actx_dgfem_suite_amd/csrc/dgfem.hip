@@ -1104,7 +1104,8 @@ namespace {
 // The mixed wave kernels on the direct flux LIFT the velocity flux in
 // reference normals: refused on a geometry that does not allow it.
 int check_mixed_geometry(const dg_plan* p, int32_t precision, const char* name) {
-  if (p->dim != 2 && mixed_direct_flux_of(p->equation, precision, p->order) && !p->ref_normals_ok)
+  if (p->dim != 2 && p->equation == DG_EQ_WAVE && mixed_direct_flux_of(p->equation, precision, p->order) &&
+      !p->ref_normals_ok)
     return fail(DG_ERR_UNSUPPORTED,
                 std::string(name) +
                     ": the mixed wave kernel LIFTs its velocity flux in reference normals, and "

@@ -284,7 +284,10 @@ constexpr bool mixed_maxwell_bform() {
 // Whether the FP32 wave kernel takes the FP64 kernel's direct flux with the
 // velocity flux LIFTed in reference normals and in-tile neighbours from the
 // staged state (rhs_mfma.h): its LIFT tiles in the direct k order, the
-// exterior map from ext_d.
+// exterior map from ext_d.  FP32 Maxwell takes it where the FP64 kernel
+// does (P2, P4), with the flux LIFTed per field: measured on MI355X, P2 -8%
+// (41^3 x 6 tets) and -9% (30^3 x 6), P4 -9% on 384k tets and even on 94k
+// (136 against 172 registers at P2, so three waves per SIMD instead of two).
 //
 // BF16x3 (four k per lane per instruction) takes it at P3, where the FP64
 // kernel's k order is packed: instruction kq, lane group g and its i-th k hold
@@ -293,9 +296,10 @@ constexpr bool mixed_maxwell_bform() {
 __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int order) {
   // BF16x3 P2 measured 30% slower on the direct flux (its registers cost a
   // wave per SIMD); P3 10% faster
-  return eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
-         (prec == DG_PREC_FP32 ||
-          (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4));
+  return (eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
+          (prec == DG_PREC_FP32 ||
+           (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4))) ||
+         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32 && direct_flux_of(eq, order));
 }
 // Instructions of the direct LIFT contraction and the face-node pair of k
 // index i of lane group g in instruction kq (-1: padding).
@@ -357,8 +361,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // P3 -10%; at P2 +5% and at P4 even, where its LDS costs a block per CU)
   // the tile's face geometry and exterior map held for the whole tile (DFG):
   // P3, and FP32 P2 with the geometry formed from rx (FP32 P2 -11.5%; FP32
-  // P4 +16%)
-  constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32));
+  // wave P4 +16%), and FP32 Maxwell (P4: -13% to -21% against the geometry
+  // loaded per k-step)
+  constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32) ||
+                               EQ == DG_EQ_MAXWELL);
   constexpr bool FGM = !DFM && mixed_face_geo_lds<EQ, PREC, ORDER>();
   constexpr bool FGL = DFG || FGM;   // the tile's face geometry in LDS
   __shared__ double2 s_fgeo[kBlock / 64][FGL ? 128 : 1];
@@ -407,13 +413,14 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     double2* const fgl = s_fgeo[tid >> 6];
     constexpr bool MAPA = DFG && PREC == DG_PREC_FP32;   // map entries preloaded
     int32_t ext_all[MAPA ? KSD : 1];
-    // face geometry in LDS: on the direct flux, BF16x3 copies the tile's
-    // table (forming it from rx measured +1.7% there) and FP32 forms it from
-    // rx below (the plan refuses that kernel on other geometry); off it
-    // (FGM), formed from rx where the plan found Fscale n = rx^T c_f on
-    // every face, else copied
-    constexpr bool DFG_RX = DFG && PREC == DG_PREC_FP32;
-    const bool geo_rx = DFG_RX || (FGM && a.face_geo_rx != 0);
+    // face geometry in LDS: on the direct flux, BF16x3 wave copies the
+    // tile's table (forming it from rx measured +1.7% there) and FP32 wave
+    // forms it from rx below (the plan refuses that kernel on other
+    // geometry); FP32 Maxwell on the direct flux and the kernels off it
+    // (FGM) form it from rx where the plan found Fscale n = rx^T c_f on
+    // every face, else copy it
+    constexpr bool DFG_RX = DFG && PREC == DG_PREC_FP32 && EQ == DG_EQ_WAVE;
+    const bool geo_rx = DFG_RX || ((FGM || (DFG && EQ != DG_EQ_WAVE)) && a.face_geo_rx != 0);
     if constexpr (FGL) {
       if (!geo_rx) {
         const double2* src = reinterpret_cast<const double2*>(a.fgeo) + tile * 128;
@@ -436,9 +443,9 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     for (int k = 0; k < 9; ++k) rx[k] = TT ? a.vgeo[(tile * 9 + k) * 16 + n] : a.geo[el * kGeo + k];
     if constexpr (FGL) if (geo_rx) {
       // the tile's face geometry from rx, Fscale n = rx^T c_g for face g of
-      // element n (lane (g, n)), as the FP64 kernel forms it; the plan
-      // refuses this kernel on geometry that does not satisfy it.  Measured
-      // on MI355X: FP32 wave P3 -8% (162k tets), -13% (216k tets)
+      // element n (lane (g, n)), as the FP64 kernel forms it, on geometry
+      // that satisfies it (geo_rx above).  Measured on MI355X: FP32 wave P3
+      // -8% (162k tets), -13% (216k tets)
       double m[3];
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
@@ -693,6 +700,50 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
       for (int c = 0; c < NF; ++c)
         rsq[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.q[c], (short)0, field_bytes, kBufferWord3);
+      if constexpr (EQ != DG_EQ_WAVE) {
+        // FP32 Maxwell: at k-step ks lane (g, n) evaluates the FP64 flux at
+        // face-node pair direct_pair_of(ks, g) of element n and feeds every
+        // field's Fscale-scaled flux to its own LIFT B operand; the PEC
+        // boundary state is the lane's own trace with E negated (the k-step
+        // loop rolled: the form measured)
+#pragma unroll 1
+        for (int ks = 0; ks < KSD; ++ks) {
+          const int fp = direct_pair_of(EQ, ORDER, ks, g);
+          const bool fok = fp >= 0;
+          const int f = fok ? fp / NFP : ks / ((NFP + 3) / 4);
+          const int vloc = s_fmask[fok ? fp : f * NFP];
+          static_assert(DFG && MAPA, "FP32 Maxwell holds the tile's geometry and map");
+          const double2 g01 = fgl[2 * (f * 16 + n)];
+          const double2 g23 = fgl[2 * (f * 16 + n) + 1];
+          const int32_t ext = ext_all[ks];
+          const bool loc = ext >= kInTile;
+          const bool glob = ext >= 0 && !loc;
+          const int lofs = loc ? ((ext >> 8) & 15) * NP + (ext & 255) : n * NP + vloc;
+          const uint32_t boff = glob ? (uint32_t)ext * 4u : field_bytes;
+          const float wl = glob ? 0.f : 1.f;
+          double qm[NF], qp[NF], out[NF];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) {
+            qm[c] = stg[c * 16 * NP + n * NP + vloc];
+            const float gv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsq[c], boff, 0, 0));
+            qp[c] = (double)__builtin_fmaf(stg[c * 16 * NP + lofs], wl, gv);
+          }
+#pragma unroll
+          for (int c = 0; c < NF; ++c)
+            if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
+          const double nrm[3] = {g01.x, g01.y, g23.x};
+          Flux::flux(qm, qp, nrm, c0, out);
+          float bv[NF][1];
+#pragma unroll
+          for (int c = 0; c < NF; ++c) bv[c][0] = fok ? (float)(g23.y * out[c]) : 0.f;
+#pragma unroll
+          for (int b = 0; b < MB; ++b) {
+            const uint32_t* av = &s_lift[(b * KSD + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < NF; ++c) acc[c][b] = M::mma(av, bv[c], acc[c][b]);
+          }
+        }
+      } else {
       // Two forms of the same loop: per k-step with one k per lane (FP32 P2,
       // P3), and per instruction over its KG k (BF16x3, and FP32 P4).  The
       // compiler schedules them differently: measured on MI355X, the first
@@ -852,6 +903,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
           for (int x = 0; x < 3; ++x)
             acc[1 + x][b][v] = (float)(rx[3 * x] * t0 + rx[3 * x + 1] * t1 + rx[3 * x + 2] * t2);
         }
+      }
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // reads of the staged tile done
     } else {
 
