@@ -1101,11 +1101,18 @@ const char* dg_last_error_string(void) { return g_last_error.c_str(); }
 
 namespace {
 
-// The mixed wave kernels on the direct flux LIFT the velocity flux in
-// reference normals: refused on a geometry that does not allow it.
+// The mixed kernels on the direct flux gather neighbours by buffer loads
+// with 32-bit byte offsets: refused where an FP32 field reaches 4 GiB.  The
+// wave ones also LIFT the velocity flux in reference normals: refused on a
+// geometry that does not allow it.
 int check_mixed_geometry(const dg_plan* p, int32_t precision, const char* name) {
-  if (p->dim != 2 && p->equation == DG_EQ_WAVE && mixed_direct_flux_of(p->equation, precision, p->order) &&
-      !p->ref_normals_ok)
+  if (p->dim == 2 || !mixed_direct_flux_of(p->equation, precision, p->order)) return DG_OK;
+  if (p->nelem * p->np * (int64_t)sizeof(float) >= ((int64_t)1 << 32))
+    return fail(DG_ERR_UNSUPPORTED,
+                std::string(name) +
+                    ": each FP32 field must stay below 4 GiB for this mixed kernel (32-bit "
+                    "buffer offsets); use the FP64 form");
+  if (p->equation == DG_EQ_WAVE && !p->ref_normals_ok)
     return fail(DG_ERR_UNSUPPORTED,
                 std::string(name) +
                     ": the mixed wave kernel LIFTs its velocity flux in reference normals, and "
@@ -1394,10 +1401,15 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       free_plan(p);
       return rc;
     }
-    if (direct_flux_of(d->equation, d->order)) {
+    // the direct map: for the FP64 kernel where it runs the direct flux, and
+    // for the FP32 mixed kernel alone where only that one does (Euler
+    // P1-P3), with the in-tile entries, since that kernel stages its tile
+    const bool mixed_only = !direct_flux_of(d->equation, d->order) &&
+                            mixed_direct_flux_of(d->equation, DG_PREC_FP32, d->order);
+    if (direct_flux_of(d->equation, d->order) || mixed_only) {
       std::vector<int32_t> ext_d;
       build_direct_map(k, d->equation, d->order, d->ext_map, ext_d);
-      if (intile_neighbours_of(d->equation, d->order)) {
+      if (intile_neighbours_of(d->equation, d->order) || mixed_only) {
         // neighbours inside the tile point into the staged state
         const int ksd = direct_ksteps_of(d->equation, d->order);
         const int64_t nt = (k + 15) / 16;

@@ -288,6 +288,11 @@ constexpr bool mixed_maxwell_bform() {
 // does (P2, P4), with the flux LIFTed per field: measured on MI355X, P2 -8%
 // (41^3 x 6 tets) and -9% (30^3 x 6), P4 -9% on 384k tets and even on 94k
 // (136 against 172 registers at P2, so three waves per SIMD instead of two).
+// FP32 Euler takes it at every order, where the plan builds the direct map
+// (with its in-tile entries) for this kernel alone at P1-P3: P1 -6% (56^3 x
+// 6 tets) and -4% (40^3 x 6; 128 against 164 registers), P2 -1.5%, P3 -16%
+// (33^3 x 6) and -19% (45^3 x 6; 236 against 276 registers, two waves per
+// SIMD instead of one), P4 -3%.
 //
 // BF16x3 (four k per lane per instruction) takes it at P3, where the FP64
 // kernel's k order is packed: instruction kq, lane group g and its i-th k hold
@@ -299,7 +304,8 @@ __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int or
   return (eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
           (prec == DG_PREC_FP32 ||
            (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4))) ||
-         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32 && direct_flux_of(eq, order));
+         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32 && direct_flux_of(eq, order)) ||
+         (eq == DG_EQ_EULER && prec == DG_PREC_FP32);
 }
 // Instructions of the direct LIFT contraction and the face-node pair of k
 // index i of lane group g in instruction kq (-1: padding).
@@ -361,10 +367,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // P3 -10%; at P2 +5% and at P4 even, where its LDS costs a block per CU)
   // the tile's face geometry and exterior map held for the whole tile (DFG):
   // P3, and FP32 P2 with the geometry formed from rx (FP32 P2 -11.5%; FP32
-  // wave P4 +16%), and FP32 Maxwell (P4: -13% to -21% against the geometry
-  // loaded per k-step)
+  // wave P4 +16%), and FP32 Euler and Maxwell (Maxwell P4: -13% to -21%
+  // against the geometry loaded per k-step)
   constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32) ||
-                               EQ == DG_EQ_MAXWELL);
+                               EQ != DG_EQ_WAVE);
   constexpr bool FGM = !DFM && mixed_face_geo_lds<EQ, PREC, ORDER>();
   constexpr bool FGL = DFG || FGM;   // the tile's face geometry in LDS
   __shared__ double2 s_fgeo[kBlock / 64][FGL ? 128 : 1];
@@ -416,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
     // face geometry in LDS: on the direct flux, BF16x3 wave copies the
     // tile's table (forming it from rx measured +1.7% there) and FP32 wave
     // forms it from rx below (the plan refuses that kernel on other
-    // geometry); FP32 Maxwell on the direct flux and the kernels off it
+    // geometry); FP32 Euler and Maxwell on the direct flux and the kernels off it
     // (FGM) form it from rx where the plan found Fscale n = rx^T c_f on
     // every face, else copy it
     constexpr bool DFG_RX = DFG && PREC == DG_PREC_FP32 && EQ == DG_EQ_WAVE;
@@ -701,18 +707,19 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
       for (int c = 0; c < NF; ++c)
         rsq[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.q[c], (short)0, field_bytes, kBufferWord3);
       if constexpr (EQ != DG_EQ_WAVE) {
-        // FP32 Maxwell: at k-step ks lane (g, n) evaluates the FP64 flux at
-        // face-node pair direct_pair_of(ks, g) of element n and feeds every
-        // field's Fscale-scaled flux to its own LIFT B operand; the PEC
-        // boundary state is the lane's own trace with E negated (the k-step
-        // loop rolled: the form measured)
+        // FP32 Euler and Maxwell: at k-step ks lane (g, n) evaluates the
+        // FP64 flux at face-node pair direct_pair_of(ks, g) of element n and
+        // feeds every field's Fscale-scaled flux to its own LIFT B operand;
+        // the boundary state is Flux::boundary of the lane's own trace (for
+        // Maxwell's PEC walls, the trace with E negated) (the k-step loop
+        // rolled: the form measured)
 #pragma unroll 1
         for (int ks = 0; ks < KSD; ++ks) {
           const int fp = direct_pair_of(EQ, ORDER, ks, g);
           const bool fok = fp >= 0;
           const int f = fok ? fp / NFP : ks / ((NFP + 3) / 4);
           const int vloc = s_fmask[fok ? fp : f * NFP];
-          static_assert(DFG && MAPA, "FP32 Maxwell holds the tile's geometry and map");
+          static_assert(DFG && MAPA, "FP32 Euler and Maxwell hold the tile's geometry and map");
           const double2 g01 = fgl[2 * (f * 16 + n)];
           const double2 g23 = fgl[2 * (f * 16 + n) + 1];
           const int32_t ext = ext_all[ks];
@@ -728,10 +735,17 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
             const float gv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsq[c], boff, 0, 0));
             qp[c] = (double)__builtin_fmaf(stg[c * 16 * NP + lofs], wl, gv);
           }
-#pragma unroll
-          for (int c = 0; c < NF; ++c)
-            if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
           const double nrm[3] = {g01.x, g01.y, g23.x};
+          if constexpr (Flux::mirror_sign(0) != 0) {
+#pragma unroll
+            for (int c = 0; c < NF; ++c)
+              if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
+          } else {
+            double qb[NF];
+            Flux::boundary(qm, nrm, qb);
+#pragma unroll
+            for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+          }
           Flux::flux(qm, qp, nrm, c0, out);
           float bv[NF][1];
 #pragma unroll

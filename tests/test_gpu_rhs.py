@@ -409,15 +409,16 @@ def test_rhs_single_partial_tile(equation, order, variant, device):
             assert max(field_errors(m, ref)) < 1e-3, prec
 
 
-@pytest.mark.parametrize("equation,order", [("euler", 1), ("euler", 2), ("maxwell", 2),
-                                            ("maxwell", 3), ("maxwell", 4), ("wave", 3)])
+@pytest.mark.parametrize("equation,order", [("euler", 1), ("euler", 2), ("euler", 3),
+                                            ("maxwell", 2), ("maxwell", 3), ("maxwell", 4),
+                                            ("wave", 3)])
 def test_mfma_face_geometry_from_rx_or_table(equation, order, device):
     """The MFMA kernel forms its tiles' face geometry from rx where every
     face satisfies Fscale n_f = rx^T c_f, and copies the plan's table where
     one does not (Euler and Maxwell, which take any geometry on this kernel).
     Both match a host evaluation of the same fused forms on their geometry,
     and so do the mixed forms (their kernels form or copy it the same way,
-    FP32 Maxwell P2 and P4 on the direct flux); the wave plan on the altered
+    FP32 Euler P1-P4 and Maxwell P2 and P4 on the direct flux); the wave plan on the altered
     geometry goes to the LDS-staged kernel."""
     print(f"[case] face geometry from rx or table {equation} P{order}", flush=True)
     from oracle import dg_cpu
