@@ -284,17 +284,19 @@ constexpr bool mixed_maxwell_bform() {
 // Whether the FP32 wave kernel takes the FP64 kernel's direct flux with the
 // velocity flux LIFTed in reference normals and in-tile neighbours from the
 // staged state (rhs_mfma.h): its LIFT tiles in the direct k order, the
-// exterior map from ext_d.  FP32 Maxwell takes it where the FP64 kernel
-// does (P2, P4), with the flux LIFTed per field: measured on MI355X, P2 -8%
-// (41^3 x 6 tets) and -9% (30^3 x 6), P4 -9% on 384k tets and even on 94k
-// (136 against 172 registers at P2, so three waves per SIMD instead of two).
-// FP32 Euler takes it at every order, where the plan builds the direct map
-// (with its in-tile entries) for this kernel alone at P1-P3: P1 -6% (56^3 x
-// 6 tets) and -4% (40^3 x 6; 128 against 164 registers), P2 -1.5%, P3 -16%
-// (33^3 x 6) and -19% (45^3 x 6; 236 against 276 registers, two waves per
-// SIMD instead of one), P4 -3%.
+// exterior map from ext_d.  FP32 Euler and Maxwell take it at every order,
+// with the flux LIFTed per field; where the FP64 kernel does not run the
+// direct flux (Euler P1-P3, Maxwell P1, P3) the plan builds the direct map,
+// with its in-tile entries, for this kernel alone.  Measured on MI355X:
+// Maxwell P1 -4% (56^3 x 6 tets, 128 against 168 registers), P2 -8% (41^3 x
+// 6) and -9% (30^3 x 6; 136 against 172 registers, so three waves per SIMD
+// instead of two), P3 -20% (25^3 and 33^3 x 6) and -25% (45^3 x 6; 228
+// against 320 registers, two waves per SIMD instead of one), P4 -9% on 384k
+// tets and even on 94k; Euler P1 -6% (56^3 x 6 tets) and -4% (40^3 x 6; 128
+// against 164 registers), P2 -1.5%, P3 -16% (33^3 x 6) and -19% (45^3 x 6;
+// 236 against 276 registers, two waves per SIMD instead of one), P4 -3%.
 //
-// BF16x3 (four k per lane per instruction) takes it at P3, where the FP64
+// BF16x3 wave (four k per lane per instruction) takes it at P3, where the FP64
 // kernel's k order is packed: instruction kq, lane group g and its i-th k hold
 // face-node pair 16 kq + 4 g + i, which is slot (4 kq + g, lane 16 i + n) of
 // the FP64 kernel's exterior map ext_d.
@@ -304,7 +306,7 @@ __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int or
   return (eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
           (prec == DG_PREC_FP32 ||
            (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4))) ||
-         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32 && direct_flux_of(eq, order)) ||
+         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32) ||
          (eq == DG_EQ_EULER && prec == DG_PREC_FP32);
 }
 // Instructions of the direct LIFT contraction and the face-node pair of k

@@ -418,7 +418,7 @@ def test_mfma_face_geometry_from_rx_or_table(equation, order, device):
     one does not (Euler and Maxwell, which take any geometry on this kernel).
     Both match a host evaluation of the same fused forms on their geometry,
     and so do the mixed forms (their kernels form or copy it the same way,
-    FP32 Euler P1-P4 and Maxwell P2 and P4 on the direct flux); the wave plan on the altered
+    FP32 Euler and Maxwell on the direct flux); the wave plan on the altered
     geometry goes to the LDS-staged kernel."""
     print(f"[case] face geometry from rx or table {equation} P{order}", flush=True)
     from oracle import dg_cpu
