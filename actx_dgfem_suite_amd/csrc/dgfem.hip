@@ -1402,9 +1402,9 @@ int dg_plan_create(const dg_plan_desc* d, dg_plan** out) {
       return rc;
     }
     // the direct map: for the FP64 kernel where it runs the direct flux, and
-    // for the FP32 mixed kernel alone where only that one does (Euler
-    // P1-P3, Maxwell P1, P3), with the in-tile entries, since that kernel
-    // stages its tile
+    // for the FP32 mixed kernel alone where only that one does (wave P1,
+    // Euler P1-P3, Maxwell P1, P3), with the in-tile entries, since that
+    // kernel stages its tile
     const bool mixed_only = !direct_flux_of(d->equation, d->order) &&
                             mixed_direct_flux_of(d->equation, DG_PREC_FP32, d->order);
     if (direct_flux_of(d->equation, d->order) || mixed_only) {

@@ -284,10 +284,13 @@ constexpr bool mixed_maxwell_bform() {
 // Whether the FP32 wave kernel takes the FP64 kernel's direct flux with the
 // velocity flux LIFTed in reference normals and in-tile neighbours from the
 // staged state (rhs_mfma.h): its LIFT tiles in the direct k order, the
-// exterior map from ext_d.  FP32 Euler and Maxwell take it at every order,
-// with the flux LIFTed per field; where the FP64 kernel does not run the
-// direct flux (Euler P1-P3, Maxwell P1, P3) the plan builds the direct map,
-// with its in-tile entries, for this kernel alone.  Measured on MI355X:
+// exterior map from ext_d.  FP32 wave takes it at P1 too, where the FP64
+// kernel does not (-14% on 40^3 x 6 tets, -15% on 56^3 x 6, -22% on 70^3 x 6,
+// with the tile's face geometry and map held; +28% to even without).  FP32
+// Euler and Maxwell take it at every order, with the flux LIFTed per field.
+// Where the FP64 kernel does not run the direct flux (wave P1, Euler P1-P3,
+// Maxwell P1, P3) the plan builds the direct map, with its in-tile entries,
+// for this kernel alone.  Measured on MI355X:
 // Maxwell P1 -4% (56^3 x 6 tets, 128 against 168 registers), P2 -8% (41^3 x
 // 6) and -9% (30^3 x 6; 136 against 172 registers, so three waves per SIMD
 // instead of two), P3 -20% (25^3 and 33^3 x 6) and -25% (45^3 x 6; 228
@@ -303,7 +306,7 @@ constexpr bool mixed_maxwell_bform() {
 __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int order) {
   // BF16x3 P2 measured 30% slower on the direct flux (its registers cost a
   // wave per SIMD); P3 10% faster
-  return (eq == DG_EQ_WAVE && direct_flux_of(eq, order) &&
+  return (eq == DG_EQ_WAVE &&
           (prec == DG_PREC_FP32 ||
            (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4))) ||
          (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32) ||
@@ -368,10 +371,10 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
   // every k-step's map entry loaded at the tile's start (measured: FP32 wave
   // P3 -10%; at P2 +5% and at P4 even, where its LDS costs a block per CU)
   // the tile's face geometry and exterior map held for the whole tile (DFG):
-  // P3, and FP32 P2 with the geometry formed from rx (FP32 P2 -11.5%; FP32
-  // wave P4 +16%), and FP32 Euler and Maxwell (Maxwell P4: -13% to -21%
-  // against the geometry loaded per k-step)
-  constexpr bool DFG = DFM && (ORDER == 3 || (ORDER == 2 && PREC == DG_PREC_FP32) ||
+  // P3, and FP32 P1-P2 with the geometry formed from rx (FP32 P2 -11.5%, P1
+  // -14% to -22%; FP32 wave P4 +16%), and FP32 Euler and Maxwell (Maxwell
+  // P4: -13% to -21% against the geometry loaded per k-step)
+  constexpr bool DFG = DFM && (ORDER == 3 || (ORDER <= 2 && PREC == DG_PREC_FP32) ||
                                EQ != DG_EQ_WAVE);
   constexpr bool FGM = !DFM && mixed_face_geo_lds<EQ, PREC, ORDER>();
   constexpr bool FGL = DFG || FGM;   // the tile's face geometry in LDS

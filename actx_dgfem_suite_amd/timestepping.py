@@ -61,7 +61,7 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
 # stage kernels).  FP64, at least 1%: wave P1 1.25x, P3 1.19x, Maxwell P1
 # 1.24x, P2 1.01x, Euler P2 1.02x; wave P2 1.00x, P4 0.98x, Euler P1 0.93x,
 # P3 0.95x, P4 1.01x, Maxwell P3 1.00x, P4 1.00x.  Mixed, at least 4%: FP32
-# wave P1 1.10x, Euler P1 1.08x, P2 1.11x, Maxwell P2 1.04x, P3 1.08x (Euler
+# wave P1 1.15x, Euler P1 1.08x, P2 1.11x, Maxwell P2 1.04x, P3 1.08x (Euler
 # P3 1.03x, P4 0.96x, Maxwell P1 0.99x, P4 0.97x); BF16x3 wave P1 1.06x, P2
 # 1.04x, P4 1.06x on 584k tets (1.00x on 118k); elsewhere 0.63x-1.02x.
 # Unfused, the stage kernel re-reads the state and RHS that the RHS kernel

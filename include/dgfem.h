@@ -140,14 +140,13 @@ int dg_rhs(const dg_plan* plan, int32_t precision,
  * included) and the combination of reference into physical derivatives run
  * in FP64.  Triangle plans (dim = 2) take FP32 state and RHS under both
  * precisions and compute everything in FP64.  The kernels that run the
- * direct flux (wave FP32 P2-P4 and BF16X3 P3, Euler and Maxwell FP32
- * P1-P4) return DG_ERR_UNSUPPORTED where an FP32 field reaches
- * 4 GiB (32-bit buffer offsets); the wave ones LIFT the velocity flux in reference
- * normals and also refuse a plan whose face geometry is not the affine
- * image of the reference element's (Fscale n != rx^T c_f).  The same holds
- * for dg_rhs_lserk_stage_mixed.  It has no counterpart in the
- * reference, whose array contexts evaluate the operator in the state's own
- * precision. */
+ * direct flux (FP32 at every order, BF16X3 wave P3) return
+ * DG_ERR_UNSUPPORTED where an FP32 field reaches 4 GiB (32-bit buffer
+ * offsets); the wave ones LIFT the velocity flux in reference normals and
+ * also refuse a plan whose face geometry is not the affine image of the
+ * reference element's (Fscale n != rx^T c_f).  The same holds for
+ * dg_rhs_lserk_stage_mixed.  It has no counterpart in the reference, whose
+ * array contexts evaluate the operator in the state's own precision. */
 int dg_rhs_mixed(const dg_plan* plan, int32_t precision,
                  const float* const* fields_in, float* const* fields_out,
                  void* stream);
