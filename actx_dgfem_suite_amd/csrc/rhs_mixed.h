@@ -211,39 +211,39 @@ struct MixedShape {
 
 constexpr int kMixedFluxRow = 65;
 
+// The next three choices hold off the direct flux, i.e. for BF16x3 but
+// wave P3: every FP32 kernel runs the direct flux (mixed_direct_flux_of),
+// which always stages the tile and holds its face geometry.  (Before it
+// did, FP32 gained from each of them as well.)
+//
 // Whether the volume phase keeps the tile's FP32 state in LDS (NF * 16 * Np
 // floats per wave) for the flux producer's own traces, as the FP64 MFMA
 // kernel does.  On where it fits without lowering the blocks per CU that the
-// registers allow and measured faster on MI355X: FP32 wave P1-P4 -4% to -8%,
-// Euler P1 -7%, P2 -16%, P4 -3%, Maxwell P1 -13%, P2 -4%, P3 -16%; BF16x3
-// wave P1 -3%, Euler P1-P3 -2% to -5% (BF16x3 Maxwell P1 and P3 were 13% and
-// 5% slower with it).
+// registers allow and measured faster on MI355X: BF16x3 wave P1 -3%, Euler
+// P1-P3 -2% to -5% (BF16x3 Maxwell P1 and P3 were 13% and 5% slower with
+// it).
 template <int EQ, int PREC, int ORDER>
 constexpr bool mixed_stage_own_traces() {
-  if constexpr (PREC == DG_PREC_FP32)
-    return !((EQ == DG_EQ_EULER && ORDER == 3) || (EQ == DG_EQ_MAXWELL && ORDER == 4));
   return (EQ == DG_EQ_WAVE && ORDER == 1) || (EQ == DG_EQ_EULER && ORDER <= 3);
 }
 
 // Whether the flux producer gathers without a branch (a boundary node loads
 // its own trace and takes the boundary state by selection), so that all items
 // are one basic block and their loads are in flight together, as in the FP64
-// MFMA kernel.  Measured on MI355X, bit-identical results: FP32 wave P3 -12%,
-// P4 -17%, Euler P3 -7%, P4 -3%, Maxwell P1-P4 -7% to -11%; BF16x3 wave P1
+// MFMA kernel.  Measured on MI355X, bit-identical results: BF16x3 wave P1
 // -7%, P4 -15%, Euler P1-P4 -2% to -10%, Maxwell P1 -6%, P2 -2%.  Off where
-// it was slower: FP32 wave P1 (+2%), P2 (+15%) and Euler P2 (+25%, where its
-// registers cost a wave per SIMD); BF16x3 Maxwell P3 (+25%) and P4 (+17%).
+// it was slower: BF16x3 Maxwell P3 (+25%) and P4 (+17%).
 template <int EQ, int PREC, int ORDER>
 constexpr bool mixed_branch_free_gathers() {
-  if constexpr (PREC == DG_PREC_FP32)
-    return !((EQ == DG_EQ_WAVE && ORDER <= 2) || (EQ == DG_EQ_EULER && ORDER == 2));
   return !(EQ == DG_EQ_MAXWELL && ORDER >= 3);
 }
 
 // Whether the kernel reads rx, the face geometry and the exterior map from
 // the FP64 MFMA kernel's tile-ordered tables, producing the flux in that
 // kernel's chunk order (one contiguous load per slot), instead of from the
-// element-ordered geometry and map.  Measured on MI355X, bit-identical
+// element-ordered geometry and map (on the direct flux, which has its own
+// map and face geometry, it picks where rx comes from).  Measured on MI355X
+// off the direct flux, bit-identical
 // results: FP32 wave P1 -11%, P2 -3%, Euler P4 -5%, Maxwell P1 -12%, P2 -3%;
 // BF16x3 wave P2 -5%, P3 -11%, Euler P2-P4 -3% to -11%, Maxwell P1 -12%,
 // P2 -19%.  Off where it was even or slower: FP32 Euler P3 +61%, BF16x3
@@ -325,20 +325,14 @@ __host__ __device__ constexpr int mixed_pair_of(int eq, int prec, int order, int
 // Whether the flux producer off the direct flux reads its face geometry
 // from a per-wave LDS copy of the tile's, formed from rx (Fscale n = rx^T
 // c_f, where the plan finds it on every face; else copied from the table)
-// instead of two global loads per flux item.  Measured on MI355X, the cases
-// that gained: FP32 Maxwell P1 -24%, wave P1 -16%, Euler P3 -7%, Maxwell P2
-// -4.6%, P4 -4.2%, Euler P1 -2%; BF16x3 Maxwell P1 -17%, wave P2 -7.5%,
-// Maxwell P4 -6.6%, Euler P4 -3.7%, Maxwell P2 -2%.  Elsewhere up to +64%
-// (FP32 Maxwell P3; BF16x3 wave P1 +14%), where its LDS or registers cost a
-// wave.
+// instead of two global loads per flux item (off the direct flux: BF16x3).
+// Measured on MI355X, the cases that gained: BF16x3 Maxwell P1 -17%, wave
+// P2 -7.5%, Maxwell P4 -6.6%, Euler P4 -3.7%, Maxwell P2 -2%.  Elsewhere up
+// to +14% (BF16x3 wave P1), where its LDS or registers cost a wave.
 template <int EQ, int PREC, int ORDER>
 constexpr bool mixed_face_geo_lds() {
-  if constexpr (PREC == DG_PREC_FP32)
-    return (EQ == DG_EQ_WAVE && ORDER == 1) || (EQ == DG_EQ_EULER && (ORDER == 1 || ORDER == 3)) ||
-           (EQ == DG_EQ_MAXWELL && ORDER != 3);
-  else
-    return (EQ == DG_EQ_WAVE && ORDER == 2) || (EQ == DG_EQ_EULER && ORDER == 4) ||
-           (EQ == DG_EQ_MAXWELL && ORDER != 3);
+  return (EQ == DG_EQ_WAVE && ORDER == 2) || (EQ == DG_EQ_EULER && ORDER == 4) ||
+         (EQ == DG_EQ_MAXWELL && ORDER != 3);
 }
 
 // STAGE: instead of storing the FP32 RHS, the kernel applies the stage
