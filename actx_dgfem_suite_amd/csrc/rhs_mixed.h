@@ -299,18 +299,19 @@ constexpr bool mixed_maxwell_bform() {
 // against 164 registers), P2 -1.5%, P3 -16% (33^3 x 6) and -19% (45^3 x 6;
 // 236 against 276 registers, two waves per SIMD instead of one), P4 -3%.
 //
-// BF16x3 wave (four k per lane per instruction) takes it at P3, where the FP64
-// kernel's k order is packed: instruction kq, lane group g and its i-th k hold
-// face-node pair 16 kq + 4 g + i, which is slot (4 kq + g, lane 16 i + n) of
-// the FP64 kernel's exterior map ext_d.
+// BF16x3 (four k per lane per instruction) takes it at P3, where the k order
+// is packed: instruction kq, lane group g and its i-th k hold face-node pair
+// 16 kq + 4 g + i, which is slot (4 kq + g, lane 16 i + n) of the exterior
+// map ext_d.  Measured on MI355X: wave P3 -9%, Euler P3 -9%, Maxwell P3
+// -38% (33^3 x 6 tets) and -30% (45^3 x 6; 252 against 292 registers, 63
+// against 93 KB of LDS, two waves per SIMD instead of one); at P1-P2 Euler
+// +20-24% and Maxwell +35-132%, where its registers cost waves.
 __host__ __device__ constexpr bool mixed_direct_flux_of(int eq, int prec, int order) {
-  // BF16x3 P2 measured 30% slower on the direct flux (its registers cost a
-  // wave per SIMD); P3 10% faster
-  return (eq == DG_EQ_WAVE &&
-          (prec == DG_PREC_FP32 ||
-           (order == 3 && packed_direct_of(eq, order) && bf16_kgroup_of(eq, order) == 4))) ||
-         (eq == DG_EQ_MAXWELL && prec == DG_PREC_FP32) ||
-         (eq == DG_EQ_EULER && prec == DG_PREC_FP32);
+  // every FP32 kernel; BF16x3 at P3 (wave P2 measured 30% slower on it),
+  // whose k order is packed with four k per lane
+  return prec == DG_PREC_FP32 ||
+         (prec == DG_PREC_BF16X3 && order == 3 && packed_direct_of(eq, order) &&
+          bf16_kgroup_of(eq, order) == 4);
 }
 // Instructions of the direct LIFT contraction and the face-node pair of k
 // index i of lane group g in instruction kq (-1: padding).
@@ -706,12 +707,67 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
       for (int c = 0; c < NF; ++c)
         rsq[c] = __builtin_amdgcn_make_buffer_rsrc((void*)a.q[c], (short)0, field_bytes, kBufferWord3);
       if constexpr (EQ != DG_EQ_WAVE) {
-        // FP32 Euler and Maxwell: at k-step ks lane (g, n) evaluates the
-        // FP64 flux at face-node pair direct_pair_of(ks, g) of element n and
-        // feeds every field's Fscale-scaled flux to its own LIFT B operand;
-        // the boundary state is Flux::boundary of the lane's own trace (for
-        // Maxwell's PEC walls, the trace with E negated) (the k-step loop
-        // rolled: the form measured)
+        // Euler and Maxwell: a lane evaluates the FP64 flux at each
+        // face-node pair that feeds its LIFT B operand (FP32: pair
+        // direct_pair_of(ks, g) of element n at k-step ks; BF16x3: its KG
+        // pairs per instruction) and feeds every field's Fscale-scaled flux
+        // to it; the boundary state is Flux::boundary of the lane's own
+        // trace (for Maxwell's PEC walls, the trace with E negated) (the
+        // loops rolled: the form measured)
+        if constexpr (PREC != DG_PREC_FP32) {
+          // BF16x3 (P3): per instruction, its KG face-node pairs per lane,
+          // each pair's map entry from its packed slot of ext_d
+#pragma unroll 1
+          for (int kq = 0; kq < KQD; ++kq) {
+            float bv[NF][KG];
+#pragma unroll
+            for (int i = 0; i < KG; ++i) {
+              const int fp = mixed_pair_of(EQ, PREC, ORDER, kq, g, i);
+              const bool fok = fp >= 0;
+              const int pp = fok ? fp : 0;
+              const int f = pp / NFP;
+              const int vloc = s_fmask[pp];
+              const double2 g01 = fgl[2 * (f * 16 + n)];
+              const double2 g23 = fgl[2 * (f * 16 + n) + 1];
+              const int32_t ext = a.ext_d[(tile * KSD + pp / 4) * 64 + (pp % 4) * 16 + n];
+              const bool loc = ext >= kInTile;
+              const bool glob = ext >= 0 && !loc;
+              const int lofs = loc ? ((ext >> 8) & 15) * NP + (ext & 255) : n * NP + vloc;
+              const uint32_t boff = glob ? (uint32_t)ext * 4u : field_bytes;
+              const float wl = glob ? 0.f : 1.f;
+              double qm[NF], qp[NF], out[NF];
+#pragma unroll
+              for (int c = 0; c < NF; ++c) {
+                qm[c] = stg[c * 16 * NP + n * NP + vloc];
+                const float gv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsq[c], boff, 0, 0));
+                qp[c] = (double)__builtin_fmaf(stg[c * 16 * NP + lofs], wl, gv);
+              }
+              const double nrm[3] = {g01.x, g01.y, g23.x};
+              if constexpr (Flux::mirror_sign(0) != 0) {
+#pragma unroll
+                for (int c = 0; c < NF; ++c)
+                  if (Flux::mirror_sign(c) < 0) qp[c] = ext >= 0 ? qp[c] : -qp[c];
+              } else {
+                double qb[NF];
+                Flux::boundary(qm, nrm, qb);
+#pragma unroll
+                for (int c = 0; c < NF; ++c) qp[c] = ext >= 0 ? qp[c] : qb[c];
+              }
+              Flux::flux(qm, qp, nrm, c0, out);
+#pragma unroll
+              for (int c = 0; c < NF; ++c) bv[c][i] = fok ? (float)(g23.y * out[c]) : 0.f;
+            }
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+              const uint32_t* av = &s_lift[((b * KQD + kq) * 64 + lane) * AW];
+              uint32_t aw[AW];
+#pragma unroll
+              for (int w = 0; w < AW; ++w) aw[w] = av[w];
+#pragma unroll
+              for (int c = 0; c < NF; ++c) acc[c][b] = M::mma(aw, bv[c], acc[c][b]);
+            }
+          }
+        } else {
 #pragma unroll 1
         for (int ks = 0; ks < KSD; ++ks) {
           const int fp = direct_pair_of(EQ, ORDER, ks, g);
@@ -755,6 +811,7 @@ __global__ __launch_bounds__(kBlock) void rhs_mixed_kernel(MixedArgs a) {
 #pragma unroll
             for (int c = 0; c < NF; ++c) acc[c][b] = M::mma(av, bv[c], acc[c][b]);
           }
+        }
         }
       } else {
       // Two forms of the same loop: per k-step with one k per lane (FP32 P2,

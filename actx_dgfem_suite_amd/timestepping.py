@@ -63,14 +63,15 @@ def _clone(fields: Sequence[torch.Tensor]) -> list[torch.Tensor]:
 # P3 0.95x, P4 1.01x, Maxwell P3 1.00x, P4 1.00x.  Mixed, at least 4%: FP32
 # wave P1 1.15x, Euler P1 1.08x, P2 1.11x, Maxwell P2 1.04x, P3 1.08x (Euler
 # P3 1.03x, P4 0.96x, Maxwell P1 0.99x, P4 0.97x); BF16x3 wave P1 1.06x, P2
-# 1.04x, P4 1.06x on 584k tets (1.00x on 118k); elsewhere 0.63x-1.02x.
+# 1.04x, P4 1.06x on 584k tets (1.00x on 118k), Euler P3 1.08x (Maxwell P3
+# 0.87x); elsewhere 0.63x-1.02x.
 # Unfused, the stage kernel re-reads the state and RHS that the RHS kernel
 # has just touched while they are still in the 256 MB Infinity Cache, and the
 # fused forms' stage epilogue costs registers their RHS kernels are short of.
 _FUSED_FP64 = {("wave", 1), ("wave", 3), ("euler", 2), ("maxwell", 1), ("maxwell", 2)}
 _FUSED_MIXED = {
     "fp32": {("wave", 1), ("euler", 1), ("euler", 2), ("maxwell", 2), ("maxwell", 3)},
-    "bf16x3": {("wave", 1), ("wave", 2), ("wave", 4)},
+    "bf16x3": {("wave", 1), ("wave", 2), ("wave", 4), ("euler", 3)},
 }
 
 

@@ -140,7 +140,7 @@ int dg_rhs(const dg_plan* plan, int32_t precision,
  * included) and the combination of reference into physical derivatives run
  * in FP64.  Triangle plans (dim = 2) take FP32 state and RHS under both
  * precisions and compute everything in FP64.  The kernels that run the
- * direct flux (FP32 at every order, BF16X3 wave P3) return
+ * direct flux (FP32 at every order, BF16X3 at P3) return
  * DG_ERR_UNSUPPORTED where an FP32 field reaches 4 GiB (32-bit buffer
  * offsets); the wave ones LIFT the velocity flux in reference normals and
  * also refuse a plan whose face geometry is not the affine image of the
